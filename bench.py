@@ -1,0 +1,152 @@
+"""Headline benchmark: FP64 TVD-RK3 explicit Euler step on one MI355X.
+
+Workload (plain GPU scale): unit cube, 128^3 hexes split into 6 tetrahedra
+each (12,582,912 cells), interior nodes jittered uniformly by +-15% of the
+spacing (splitmix64, seed 20260101), FP64, freestream Mach 0.5 (rho=1,
+p=1/gamma) plus a Gaussian density pulse (amplitude 0.1, sigma 0.1) at the
+centre, all six sides farfield, CFL 0.5, Rusanov + Green-Gauss + Barth.
+
+A step is one TVD-RK3 step (three stages) over all cells, dt computed on the
+device; the timed region replays captured HIP graphs of the step loop with
+all inputs resident in HBM.  Metric: cell-stage-updates/s = N * 3 * K / t.
+
+Prints one JSON line.  --gpus N>1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20260101
+
+
+def bytes_model(N, F):
+    """Algorithmic (compulsory) bytes of the two sweeps of one stage, FP64:
+    sweep 1 reads U (40N), volumes (8N), face records (80F) and writes the
+    limited slopes (120N); sweep 2 reads U, slopes, face records, volumes,
+    U_old (stages 2-3) and writes U'."""
+    sweep1 = 168 * N + 80 * F
+    sweep2_s1 = 208 * N + 80 * F        # stage 1: no U_old read
+    sweep2_s23 = 248 * N + 80 * F
+    step = 1232 * N + 480 * F
+    return sweep1, sweep2_s1, sweep2_s23, step
+
+
+def cpu_baseline(sample_n, steps):
+    """Oracle (serial restatement of the reference solver) on one host core."""
+    import netflux_amd as nf
+    from tests.cases import pulse_config, pulse_state
+    from tests.oracle_bind import Oracle
+
+    mesh = nf.Mesh.box(sample_n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
+    cfg = pulse_config()
+    ora = Oracle.from_mesh(mesh, cfg)
+    ora.set_state(pulse_state(mesh))
+    t0 = time.perf_counter()
+    ora.run(steps)
+    t = time.perf_counter() - t0
+    N = mesh.n_cells
+    return {"value": N * 3 * steps / t, "unit": "cell-stage-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, {sample_n}^3x6 = {N} cells, {steps} TVD-RK3 steps, same workload family "
+                      f"(jitter 0.15, pulse, farfield), one thread, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=128, help="hexes per side")
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--timing-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import netflux_amd as nf
+    from tests.cases import pulse_config, pulse_state
+
+    t0 = time.perf_counter()
+    mesh = nf.Mesh.box(args.n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    cfg = pulse_config(mach=0.5)
+    solver = nf.EulerSolver(cfg, mesh)
+    N = solver.N
+    F = solver.sizes.n_total_faces
+    print(f"[bench] mesh {args.n}^3x6 = {N} cells, {F} faces, setup {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+
+    solver.set_state(U0)
+    st = solver.run(args.warmup, check_every=args.graph_steps, use_graph=True)
+    # timed region: K steps, graph replays; nf_run synchronises before return
+    solver.set_state(U0)
+    solver.run(1, use_graph=False)  # first step has the zero-U_old quirk; excluded
+    t0 = time.perf_counter()
+    st = solver.run(args.steps, check_every=args.graph_steps, use_graph=True)
+    wall = time.perf_counter() - t0
+    ms_gpu = st["elapsed_ms"]
+    assert st["steps_done"] == args.steps, st
+    ms_per_step = ms_gpu / args.steps
+    value = N * 3 * args.steps / (ms_gpu / 1e3)
+    print(f"[bench] {args.steps} steps: {ms_gpu:.1f} ms (hipEvent), {wall * 1e3:.1f} ms wall",
+          file=sys.stderr, flush=True)
+
+    # per-kernel durations with HIP events on the solver's stream
+    kt = solver.time_kernels(args.timing_steps)
+    s1, s2a, s2b, step_bytes = bytes_model(N, F)
+    kern = {}
+    for name, (avg, calls) in kt.items():
+        kern[name] = {"avg_ms": avg, "calls": calls}
+    tot = {k: v["avg_ms"] * v["calls"] for k, v in kern.items()}
+    dom = max(tot, key=tot.get)
+    per_launch = {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0}.get(dom, s1)
+    achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
+                "algorithmic_bytes_per_launch": int(per_launch)}
+    step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_sample_n, args.cpu_steps)
+
+    out = {
+        "metric": "cell-stage-updates/s",
+        "value": value,
+        "unit": "cell-stage-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated jittered Kuhn mesh, Gaussian pulse)",
+        "config": {"workload": f"plain FP64 TVD-RK3 step, unit cube {args.n}^3 hex->6 tets "
+                               f"({N} cells), jitter 0.15, Mach 0.5 + pulse, farfield, CFL 0.5",
+                   "cells": N, "faces": F, "graph_steps": args.graph_steps},
+        "step_roofline_frac": round(step_frac, 4),
+        "kernels": kern,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

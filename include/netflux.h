@@ -1,0 +1,194 @@
+/*
+ * netflux.h - C ABI of the MI355X explicit Euler solver (libnetflux.so).
+ *
+ * Every entry point takes plain pointers and sizes and returns an int status
+ * (NF_OK = 0).  A solver context owns its device buffers and one HIP stream;
+ * it is not thread-safe, like the reference solver object it replaces.
+ * Arrays cross the boundary in the reference's own layouts:
+ *   - per-cell vectors are AoS [cell][5] doubles (include/containers/
+ *     DynamicContainer.hpp:147-157), in the reference's cell numbering
+ *     (element order of the input mesh);
+ *   - the gradient is [cell][5x3] column-major, offset k + 5*d
+ *     (include/SolverData.hpp:83-86, Eigen default storage).
+ */
+#ifndef NETFLUX_H
+#define NETFLUX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    NF_OK = 0,
+    NF_E_ARG = 1,         /* bad argument / unsupported option              */
+    NF_E_MESH = 2,        /* inconsistent mesh (unmatched face, bad patch)  */
+    NF_E_HIP = 3,         /* HIP runtime error                              */
+    NF_E_INVALID_DT = 4,  /* "Invalid dt calculated" (src/Solver.cpp:267)   */
+    NF_E_NONPHYSICAL = 5, /* "cells with unphysical values detected in the
+                              flux_balance field" (src/SolverData.cpp:73-78) */
+    NF_E_NOGPU = 6        /* no HIP device available                        */
+};
+
+/* ---- option enums: same meaning as include/Utilities.hpp:26-98 ---------- */
+enum { NF_EXPLICIT_EULER = 0, NF_TVD_RK3 = 1 };           /* TimeScheme      */
+enum { NF_FIRST_ORDER = 0, NF_SECOND_ORDER = 1 };         /* SpatialOrder    */
+enum { NF_LIMITER_NONE = 0, NF_LIMITER_BARTH = 1 };       /* Limiter         */
+enum { NF_FLUX_RUSANOV = 0, NF_FLUX_HLLC = 1 };           /* InviscidFluxScheme */
+enum { NF_BC_NOSLIP_WALL = 0, NF_BC_SLIP_WALL = 1, NF_BC_FARFIELD = 2 }; /* BoundaryType */
+/* arithmetic of the face sweeps */
+enum { NF_PREC_FP64 = 0,   /* everything FP64, bit-identical op order       */
+       NF_PREC_MIXED = 1   /* FP32 flux/gradient/limiter, FP64 state/RK/norms */ };
+/* cell numbering used on the device */
+enum { NF_ORDER_INPUT = 0, NF_ORDER_RCM = 1 };
+
+/* ---- primal mesh: what PrimalGrid holds after read_su2_mesh ------------
+ * (src/geometry/PrimalGrid.cpp:140-244, after salome_to_vtk_connectivity).
+ * Tetrahedra in VTK node order; boundary triangles grouped by patch, patch
+ * order = SU2 marker order, which fixes face and ghost numbering
+ * (src/geometry/FV_Grid.cpp:93-115). */
+typedef struct nf_mesh_desc {
+    uint32_t n_nodes;
+    const double *nodes;          /* [n_nodes][3]                          */
+    uint32_t n_cells;
+    const uint32_t *cells;        /* [n_cells][4] tetrahedra, VTK order    */
+    uint32_t n_patches;
+    const uint32_t *patch_n_faces;/* [n_patches] triangles per patch       */
+    const uint32_t *patch_faces;  /* [sum patch_n_faces][3]                */
+    const int32_t *patch_bc;      /* [n_patches] NF_BC_*                   */
+} nf_mesh_desc;
+
+/* Mirrors the Config keys read by src/ConfigParser.cpp:38-83 that the step
+ * depends on, plus the reference-compatibility switch for the zero U_old of
+ * the first TVD-RK3 step (src/SolverData.cpp:9, src/Solver.cpp:51). */
+typedef struct nf_params {
+    int32_t time_scheme;          /* NF_EXPLICIT_EULER | NF_TVD_RK3        */
+    int32_t spatial_order;        /* NF_FIRST_ORDER | NF_SECOND_ORDER      */
+    int32_t limiter;              /* NF_LIMITER_*                          */
+    int32_t inv_flux_scheme;      /* NF_FLUX_RUSANOV (HLLC -> NF_E_ARG)    */
+    double CFL;
+    double primvars_inf[5];       /* rho, u, v, w, p of the freestream     */
+    int32_t check_physical_validity;
+    int32_t rk3_zero_old_first_step; /* 1 = reference behaviour            */
+    int32_t precision;            /* NF_PREC_*                             */
+    int32_t cell_order;           /* NF_ORDER_*                            */
+} nf_params;
+
+/* ---- host mesh utilities (CPU only) ------------------------------------ */
+typedef struct nf_mesh nf_mesh;   /* owned primal mesh                     */
+
+/* Unit-box hexahedral grid [0,lx]x[0,ly]x[0,lz], nx*ny*nz hexes, each split
+ * into 6 tetrahedra around the (0,0,0)-(1,1,1) diagonal.  Interior nodes are
+ * moved by a uniform random offset in +-jitter*h per coordinate (splitmix64
+ * seeded by `seed`).  bump_h > 0 maps y <- yb(x) + y (1 - yb(x)/ly) with
+ * yb(x) = bump_h exp(-(x - lx/2)^2 / (2 bump_sigma^2)).
+ * Six patches in the order x-, x+, y-, y+, z-, z+ with types patch_bc[6]. */
+int nf_mesh_box(uint32_t nx, uint32_t ny, uint32_t nz,
+                double lx, double ly, double lz,
+                double jitter, uint64_t seed,
+                double bump_h, double bump_sigma,
+                const int32_t patch_bc[6], nf_mesh **out);
+int nf_mesh_from_desc(const nf_mesh_desc *desc, nf_mesh **out);
+int nf_mesh_get_desc(const nf_mesh *mesh, nf_mesh_desc *desc);
+void nf_mesh_destroy(nf_mesh *mesh);
+
+/* The finite-volume face graph built exactly as FV_Grid does
+ * (src/geometry/FV_Grid.cpp:21-267): interior faces sorted by (i,j), i<j,
+ * then patch blocks; S oriented i->j; ghost volume 0, ghost centroid
+ * reflected.  Sizes first, then caller-allocated copies. */
+typedef struct nf_fv_sizes {
+    uint32_t n_interior_cells, n_total_cells;
+    uint32_t n_interior_faces, n_total_faces;
+    uint32_t n_patches;
+} nf_fv_sizes;
+int nf_fv_sizes_of(const nf_mesh *mesh, nf_fv_sizes *sizes);
+int nf_fv_export(const nf_mesh *mesh,
+                 uint32_t *cell_ij,      /* [F][2]          */
+                 double *S,              /* [F][3]          */
+                 double *r_i,            /* [F][3]          */
+                 double *r_j,            /* [F][3]          */
+                 double *volume,         /* [N+G]           */
+                 double *centroid,       /* [N+G][3]        */
+                 uint32_t *patch_first,  /* [P]             */
+                 uint32_t *patch_nfaces, /* [P]             */
+                 double *delta_S);       /* [N+G][3]        */
+
+/* ---- solver context ------------------------------------------------------
+ * Replaces EulerSolver (include/Solver.hpp:52-79): constructed from the mesh
+ * and config, stepped with calc_timestep + step in the Driver loop order
+ * (src/Driver.cpp:31-57). */
+typedef struct nf_ctx nf_ctx;
+
+int nf_create(const nf_mesh_desc *mesh, const nf_params *params, nf_ctx **out);
+int nf_destroy(nf_ctx *ctx);
+const char *nf_last_error(const nf_ctx *ctx);
+
+/* EulerSolverData::set_freestream_values (src/SolverData.cpp:50-58):
+ * U = U_inf, primvars = V_inf, U_old = 0. */
+int nf_set_freestream(nf_ctx *ctx);
+/* Inject a state (AoS [N][5] conservative, reference cell order).  primvars
+ * become prim(U).  U_old = U_old_or_null, or zero when NULL. */
+int nf_set_state(nf_ctx *ctx, const double *U, const double *U_old_or_null);
+
+/* EulerSolver::calc_timestep (src/Solver.cpp:232-274): dt from the primvars
+ * as last set by the solver; NF_E_INVALID_DT if not finite and positive. */
+int nf_calc_timestep(nf_ctx *ctx, double *dt_out);
+/* Solver::step (src/Solver.cpp:34-52) with the given dt. */
+int nf_step(nf_ctx *ctx, double dt);
+
+/* Driver loop on the device: n_steps of (calc_timestep, step), dt kept on
+ * the device, launched as a captured HIP graph.  If residual_drop > 0 the
+ * loop stops after the first step k with r_k / r_1 <= residual_drop, where
+ * r_k = ||R_rho||_2 over interior cells at the first stage of step k;
+ * convergence is checked every check_every steps. */
+typedef struct nf_run_opts {
+    int32_t n_steps;
+    double residual_drop;         /* <= 0: run all steps                   */
+    int32_t check_every;          /* steps per graph replay (>=1)          */
+    int32_t use_graph;            /* 1: hipGraph replay, 0: eager launches */
+} nf_run_opts;
+typedef struct nf_run_stats {
+    int32_t steps_done;
+    int32_t converged;
+    double time;                  /* sum of dt                             */
+    double last_dt;
+    double res_first, res_last;   /* r_1 and r_k                           */
+    double elapsed_ms;            /* hipEvent time of the loop             */
+} nf_run_stats;
+int nf_run(nf_ctx *ctx, const nf_run_opts *opts, nf_run_stats *stats);
+/* Residual history r_k of the last nf_run / nf_step calls (up to max). */
+int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out);
+
+/* Solver::evaluate_flux_balance (src/Solver.cpp:54-81) of the current state,
+ * once, without changing U: keeps R, the gradient and the limiter for
+ * nf_get_field (the reference computes them inside every stage). */
+int nf_eval_flux_balance(nf_ctx *ctx);
+
+/* Read back (synchronising). */
+int nf_get_state(nf_ctx *ctx, double *U);            /* [N][5] conservative */
+enum { NF_FIELD_PRIM = 0,      /* [N][5] prim(U) of the current state      */
+       NF_FIELD_LIMITER = 1,   /* [N][5] psi of the last stage             */
+       NF_FIELD_GRADIENT = 2,  /* [N][15] column-major 5x3 of the last stage */
+       NF_FIELD_RESIDUAL = 3,  /* [N][5] flux balance R of the last stage  */
+       NF_FIELD_STATE_OLD = 4  /* [N][5] U_old                             */ };
+int nf_get_field(nf_ctx *ctx, int32_t which, double *out);
+
+/* Per-kernel timing of n_steps eager steps with HIP events recorded on the
+ * context stream.  names/avg_ms/calls are filled for up to max kernels. */
+typedef struct nf_kernel_time {
+    char name[48];
+    double avg_ms;
+    int32_t calls;
+} nf_kernel_time;
+int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out,
+                    int32_t max, int32_t *n_out);
+
+/* Sizes of a context (cells, faces) for callers sizing their buffers. */
+int nf_ctx_sizes(const nf_ctx *ctx, nf_fv_sizes *sizes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,112 @@
+"""ctypes binding of libnetflux.so (declarations in include/netflux.h).
+
+The library is built in-tree by ``netflux_amd/Makefile``.  Loading fails
+loudly when it is missing: there is no CPU fallback for the solver.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnetflux.so")
+
+NF_OK, NF_E_ARG, NF_E_MESH, NF_E_HIP, NF_E_INVALID_DT, NF_E_NONPHYSICAL, NF_E_NOGPU = range(7)
+NF_EXPLICIT_EULER, NF_TVD_RK3 = 0, 1
+NF_FIRST_ORDER, NF_SECOND_ORDER = 0, 1
+NF_LIMITER_NONE, NF_LIMITER_BARTH = 0, 1
+NF_FLUX_RUSANOV, NF_FLUX_HLLC = 0, 1
+NF_BC_NOSLIP_WALL, NF_BC_SLIP_WALL, NF_BC_FARFIELD = 0, 1, 2
+NF_PREC_FP64, NF_PREC_MIXED = 0, 1
+NF_ORDER_INPUT, NF_ORDER_RCM = 0, 1
+NF_FIELD_PRIM, NF_FIELD_LIMITER, NF_FIELD_GRADIENT, NF_FIELD_RESIDUAL, NF_FIELD_STATE_OLD = range(5)
+
+u32p = C.POINTER(C.c_uint32)
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32), ("nodes", f64p),
+        ("n_cells", C.c_uint32), ("cells", u32p),
+        ("n_patches", C.c_uint32), ("patch_n_faces", u32p),
+        ("patch_faces", u32p), ("patch_bc", i32p),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("time_scheme", C.c_int32), ("spatial_order", C.c_int32),
+        ("limiter", C.c_int32), ("inv_flux_scheme", C.c_int32),
+        ("CFL", C.c_double), ("primvars_inf", C.c_double * 5),
+        ("check_physical_validity", C.c_int32),
+        ("rk3_zero_old_first_step", C.c_int32),
+        ("precision", C.c_int32), ("cell_order", C.c_int32),
+    ]
+
+
+class FVSizes(C.Structure):
+    _fields_ = [("n_interior_cells", C.c_uint32), ("n_total_cells", C.c_uint32),
+                ("n_interior_faces", C.c_uint32), ("n_total_faces", C.c_uint32),
+                ("n_patches", C.c_uint32)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("n_steps", C.c_int32), ("residual_drop", C.c_double),
+                ("check_every", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps_done", C.c_int32), ("converged", C.c_int32),
+                ("time", C.c_double), ("last_dt", C.c_double),
+                ("res_first", C.c_double), ("res_last", C.c_double),
+                ("elapsed_ms", C.c_double)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("avg_ms", C.c_double), ("calls", C.c_int32)]
+
+
+# (name, restype, argtypes) of every symbol declared in include/netflux.h
+SIGNATURES = [
+    ("nf_mesh_box", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_double,
+                              i32p, C.POINTER(C.c_void_p)]),
+    ("nf_mesh_from_desc", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    ("nf_mesh_get_desc", C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
+    ("nf_mesh_destroy", None, [C.c_void_p]),
+    ("nf_fv_sizes_of", C.c_int, [C.c_void_p, C.POINTER(FVSizes)]),
+    ("nf_fv_export", C.c_int, [C.c_void_p, u32p, f64p, f64p, f64p, f64p, f64p, u32p, u32p, f64p]),
+    ("nf_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    ("nf_destroy", C.c_int, [C.c_void_p]),
+    ("nf_last_error", C.c_char_p, [C.c_void_p]),
+    ("nf_set_freestream", C.c_int, [C.c_void_p]),
+    ("nf_set_state", C.c_int, [C.c_void_p, f64p, f64p]),
+    ("nf_calc_timestep", C.c_int, [C.c_void_p, f64p]),
+    ("nf_step", C.c_int, [C.c_void_p, C.c_double]),
+    ("nf_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+    ("nf_get_residuals", C.c_int, [C.c_void_p, f64p, C.c_int32, i32p]),
+    ("nf_eval_flux_balance", C.c_int, [C.c_void_p]),
+    ("nf_get_state", C.c_int, [C.c_void_p, f64p]),
+    ("nf_get_field", C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    ("nf_time_kernels", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelTime), C.c_int32, i32p]),
+    ("nf_ctx_sizes", C.c_int, [C.c_void_p, C.POINTER(FVSizes)]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libnetflux.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C netflux_amd` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,81 @@
+// Host mesh pipeline: primal tetrahedral mesh -> finite-volume face graph in
+// the reference's face order -> per-cell device layout.
+//
+// The face graph follows FV_Grid (src/geometry/FV_Grid.cpp:21-267): interior
+// faces sorted by (i,j) with i<j, then one block per patch in patch order,
+// ghost cells numbered sequentially per patch; the geometry uses the
+// reference's tet/triangle formulas (src/geometry/GeometryUtilities.cpp:
+// 289-329) evaluated in the same operation order, so every face vector is
+// bit-identical to what the reference computes from the same nodes.
+// Matching uses buckets keyed by the smallest node of a face instead of the
+// reference's std::map (O(F) instead of O(F log F)).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nf {
+
+struct PrimalMesh {
+    std::vector<double> nodes;        // [n_nodes][3]
+    std::vector<uint32_t> cells;      // [n_cells][4] VTK order
+    std::vector<uint32_t> patch_n;    // [P]
+    std::vector<uint32_t> patch_tri;  // [sum][3]
+    std::vector<int32_t> patch_bc;    // [P]
+    uint32_t n_nodes() const { return (uint32_t)(nodes.size() / 3); }
+    uint32_t n_cells() const { return (uint32_t)(cells.size() / 4); }
+};
+
+struct FVGrid {
+    uint32_t N = 0, NT = 0, Fint = 0, F = 0;
+    std::vector<uint32_t> ij;         // [F][2]
+    std::vector<uint32_t> fnodes;     // [F][3] face element node order
+    std::vector<double> S, ri, rj;    // [F][3]
+    std::vector<double> vol;          // [NT]
+    std::vector<double> cen;          // [NT][3]
+    std::vector<uint32_t> pfirst, pn; // [P]
+    std::vector<int32_t> pbc;         // [P]
+    std::vector<uint8_t> flip;        // [F] 1 if S was negated
+    std::vector<double> dS;           // [NT][3] interior-face half |S| sums
+};
+
+// Device layout, one record per device cell (structure of arrays).
+// Slot s of a cell holds its s-th face in increasing reference face index,
+// so gathers add contributions in the reference's scatter order.
+struct CellLayout {
+    uint32_t N = 0;
+    std::vector<uint32_t> perm;   // device cell -> reference cell
+    std::vector<uint32_t> iperm;  // reference cell -> device cell
+    std::vector<uint32_t> cnodes; // [4][N] tet node ids (VTK order)
+    std::vector<uint32_t> nbr;    // [4][N] neighbour words (see below)
+    std::vector<uint32_t> fcode;  // [N] 4 x 8-bit face codes
+    std::vector<uint32_t> ghost;  // [4][N] reference ghost id (boundary slots)
+};
+
+// neighbour word: bit31 = boundary.  interior: bits 29-30 neighbour slot,
+// bits 0-28 neighbour device cell.  boundary: bits 0-7 BC type.
+constexpr uint32_t NBR_BOUNDARY = 0x80000000u;
+constexpr uint32_t NBR_INDEX_MASK = 0x1FFFFFFFu;
+// face code byte: bits 0-1,2-3,4-5 local index of face nodes a,b,c;
+// bit 6 = S negated; bit 7 = this cell is the reference owner i.
+constexpr uint32_t FC_FLIP = 0x40u;
+constexpr uint32_t FC_OWNER = 0x80u;
+
+struct MeshError {
+    std::string msg;
+};
+
+void build_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly,
+               double lz, double jitter, uint64_t seed, double bump_h,
+               double bump_sigma, const int32_t bc[6], PrimalMesh &out);
+
+// Throws MeshError on inconsistent input.
+void build_fv_grid(const PrimalMesh &m, FVGrid &g);
+
+// order: 0 = input order, 1 = reverse Cuthill-McKee.
+void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order,
+                       CellLayout &L);
+
+std::vector<uint32_t> rcm_order(const FVGrid &g);
+
+}  // namespace nf

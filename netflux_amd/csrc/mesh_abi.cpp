@@ -1,0 +1,115 @@
+// C ABI of the host mesh utilities (include/netflux.h): generators and the
+// finite-volume face graph export.  CPU only.
+#include <cstring>
+#include <new>
+
+#include "../../include/netflux.h"
+#include "mesh.hpp"
+
+struct nf_mesh {
+    nf::PrimalMesh pm;
+    nf::FVGrid fv;
+    bool fv_built = false;
+};
+
+namespace {
+int ensure_fv(nf_mesh *m) {
+    if (m->fv_built) return NF_OK;
+    try {
+        nf::build_fv_grid(m->pm, m->fv);
+    } catch (const nf::MeshError &) {
+        return NF_E_MESH;
+    }
+    m->fv_built = true;
+    return NF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int nf_mesh_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double lz,
+                double jitter, uint64_t seed, double bump_h, double bump_sigma,
+                const int32_t patch_bc[6], nf_mesh **out) {
+    if (!out || !patch_bc) return NF_E_ARG;
+    *out = nullptr;
+    if (!(lx > 0 && ly > 0 && lz > 0) || !(jitter >= 0 && jitter < 0.5)) return NF_E_ARG;
+    if (bump_h > 0 && !(bump_sigma > 0)) return NF_E_ARG;
+    for (int s = 0; s < 6; ++s)
+        if (patch_bc[s] < 0 || patch_bc[s] > 2) return NF_E_ARG;
+    nf_mesh *m = new (std::nothrow) nf_mesh;
+    if (!m) return NF_E_ARG;
+    try {
+        nf::build_box(nx, ny, nz, lx, ly, lz, jitter, seed, bump_h, bump_sigma, patch_bc, m->pm);
+    } catch (const nf::MeshError &) {
+        delete m;
+        return NF_E_ARG;
+    }
+    *out = m;
+    return NF_OK;
+}
+
+int nf_mesh_from_desc(const nf_mesh_desc *d, nf_mesh **out) {
+    if (!d || !out) return NF_E_ARG;
+    *out = nullptr;
+    nf_mesh *m = new (std::nothrow) nf_mesh;
+    if (!m) return NF_E_ARG;
+    m->pm.nodes.assign(d->nodes, d->nodes + 3 * (size_t)d->n_nodes);
+    m->pm.cells.assign(d->cells, d->cells + 4 * (size_t)d->n_cells);
+    m->pm.patch_n.assign(d->patch_n_faces, d->patch_n_faces + d->n_patches);
+    size_t nt = 0;
+    for (uint32_t p = 0; p < d->n_patches; ++p) nt += d->patch_n_faces[p];
+    m->pm.patch_tri.assign(d->patch_faces, d->patch_faces + 3 * nt);
+    m->pm.patch_bc.assign(d->patch_bc, d->patch_bc + d->n_patches);
+    *out = m;
+    return NF_OK;
+}
+
+int nf_mesh_get_desc(const nf_mesh *m, nf_mesh_desc *d) {
+    if (!m || !d) return NF_E_ARG;
+    d->n_nodes = m->pm.n_nodes();
+    d->nodes = m->pm.nodes.data();
+    d->n_cells = m->pm.n_cells();
+    d->cells = m->pm.cells.data();
+    d->n_patches = (uint32_t)m->pm.patch_n.size();
+    d->patch_n_faces = m->pm.patch_n.data();
+    d->patch_faces = m->pm.patch_tri.data();
+    d->patch_bc = m->pm.patch_bc.data();
+    return NF_OK;
+}
+
+void nf_mesh_destroy(nf_mesh *m) { delete m; }
+
+int nf_fv_sizes_of(const nf_mesh *mc, nf_fv_sizes *s) {
+    if (!mc || !s) return NF_E_ARG;
+    nf_mesh *m = const_cast<nf_mesh *>(mc);
+    int rc = ensure_fv(m);
+    if (rc) return rc;
+    const nf::FVGrid &g = m->fv;
+    *s = {g.N, g.NT, g.Fint, g.F, (uint32_t)g.pfirst.size()};
+    return NF_OK;
+}
+
+int nf_fv_export(const nf_mesh *mc, uint32_t *cell_ij, double *S, double *r_i, double *r_j,
+                 double *volume, double *centroid, uint32_t *patch_first, uint32_t *patch_nfaces,
+                 double *delta_S) {
+    if (!mc) return NF_E_ARG;
+    nf_mesh *m = const_cast<nf_mesh *>(mc);
+    int rc = ensure_fv(m);
+    if (rc) return rc;
+    const nf::FVGrid &g = m->fv;
+    auto cp = [](void *dst, const void *src, size_t bytes) {
+        if (dst) std::memcpy(dst, src, bytes);
+    };
+    cp(cell_ij, g.ij.data(), g.ij.size() * 4);
+    cp(S, g.S.data(), g.S.size() * 8);
+    cp(r_i, g.ri.data(), g.ri.size() * 8);
+    cp(r_j, g.rj.data(), g.rj.size() * 8);
+    cp(volume, g.vol.data(), g.vol.size() * 8);
+    cp(centroid, g.cen.data(), g.cen.size() * 8);
+    cp(patch_first, g.pfirst.data(), g.pfirst.size() * 4);
+    cp(patch_nfaces, g.pn.data(), g.pn.size() * 4);
+    cp(delta_S, g.dS.data(), g.dS.size() * 8);
+    return NF_OK;
+}
+
+}  // extern "C"

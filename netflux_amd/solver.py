@@ -1,0 +1,288 @@
+"""Python mirror of NetFlux's Config / EulerSolver / SolverData / Driver
+surface over the C ABI of libnetflux.so.
+
+Same names, argument meaning and error behaviour as the reference:
+  Config               include/Config.hpp:8-175 (keys of src/ConfigParser.cpp:38-83)
+  EulerSolver          include/Solver.hpp:9-79  (step, calc_timestep, get_solver_data)
+  SolverData           include/SolverData.hpp:25-73 (get_solution, get_primvars, ...)
+  Driver.solve         src/Driver.cpp:31-57
+Errors raise RuntimeError with the reference's message text
+(src/Solver.cpp:267-268, src/SolverData.cpp:73-78).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+def _check(rc, ctx=None, what=""):
+    if rc == A.NF_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        msg = A.lib().nf_last_error(ctx).decode()
+    if rc in (A.NF_E_INVALID_DT, A.NF_E_NONPHYSICAL):
+        raise RuntimeError(msg)
+    raise RuntimeError(f"{what} failed with status {rc}: {msg}")
+
+
+def _f64(a):
+    return a.ctypes.data_as(A.f64p)
+
+
+def _u32(a):
+    return a.ctypes.data_as(A.u32p)
+
+
+def _i32(a):
+    return a.ctypes.data_as(A.i32p)
+
+
+BC_NAMES = {"NoSlipWall": A.NF_BC_NOSLIP_WALL, "SlipWall": A.NF_BC_SLIP_WALL,
+            "FarField": A.NF_BC_FARFIELD}
+
+
+@dataclass
+class Config:
+    """The option set of NetFlux's Config that the explicit step depends on."""
+    time_scheme: str = "TVD_RK3"          # ExplicitEuler | TVD_RK3
+    spatial_order: str = "Second"         # First | Second
+    limiter: str = "Barth"                # NONE | Barth
+    grad_scheme: str = "GreenGauss"
+    inviscid_flux_scheme: str = "Rusanov"
+    CFL: float = 0.5
+    density_fs: float = 1.225             # standard_air defaults (include/Utilities.hpp:101-107)
+    velocity_fs: tuple = (0.0, 0.0, 0.0)
+    pressure_fs: float = 101325.0
+    n_timesteps: int = 1
+    check_physical_validity: bool = False
+    # reference quirk: U_old is zero on the first TVD-RK3 step
+    rk3_zero_old_first_step: bool = True
+    precision: str = "fp64"               # fp64 | mixed
+    cell_order: str = "input"             # input | rcm
+    # run-time state held by Config in the reference
+    delta_time: float = 0.0
+    time: float = 0.0
+    timestep: int = 0
+
+    def primvars_inf(self):
+        return [self.density_fs, *self.velocity_fs, self.pressure_fs]
+
+    def to_params(self):
+        p = A.Params()
+        p.time_scheme = {"ExplicitEuler": A.NF_EXPLICIT_EULER, "TVD_RK3": A.NF_TVD_RK3}[self.time_scheme]
+        p.spatial_order = {"First": A.NF_FIRST_ORDER, "Second": A.NF_SECOND_ORDER}[self.spatial_order]
+        p.limiter = {"NONE": A.NF_LIMITER_NONE, "Barth": A.NF_LIMITER_BARTH}[self.limiter]
+        p.inv_flux_scheme = {"Rusanov": A.NF_FLUX_RUSANOV, "HLLC": A.NF_FLUX_HLLC}[self.inviscid_flux_scheme]
+        p.CFL = self.CFL
+        for k, v in enumerate(self.primvars_inf()):
+            p.primvars_inf[k] = float(v)
+        p.check_physical_validity = int(self.check_physical_validity)
+        p.rk3_zero_old_first_step = int(self.rk3_zero_old_first_step)
+        p.precision = {"fp64": A.NF_PREC_FP64, "mixed": A.NF_PREC_MIXED}[self.precision]
+        p.cell_order = {"input": A.NF_ORDER_INPUT, "rcm": A.NF_ORDER_RCM}[self.cell_order]
+        return p
+
+
+class Mesh:
+    """Primal tetrahedral mesh (what PrimalGrid holds), owned by the library."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._desc = A.MeshDesc()
+        _check(A.lib().nf_mesh_get_desc(self._h, C.byref(self._desc)), what="nf_mesh_get_desc")
+
+    @classmethod
+    def box(cls, n, lengths=(1.0, 1.0, 1.0), jitter=0.0, seed=20260101, bump_h=0.0,
+            bump_sigma=0.2, patches=("SlipWall",) * 6):
+        nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+        bc = np.array([BC_NAMES[p] if isinstance(p, str) else int(p) for p in patches], dtype=np.int32)
+        h = C.c_void_p()
+        _check(A.lib().nf_mesh_box(nx, ny, nz, *map(float, lengths), float(jitter), int(seed),
+                                   float(bump_h), float(bump_sigma), _i32(bc), C.byref(h)),
+               what="nf_mesh_box")
+        return cls(h)
+
+    @classmethod
+    def bump_channel(cls, nx, ny, nz, h=0.05, sigma=0.2, lengths=(3.0, 1.0, 0.5)):
+        """Channel with a Gaussian floor bump; x-planes FarField, others SlipWall."""
+        return cls.box((nx, ny, nz), lengths=lengths, bump_h=h, bump_sigma=sigma,
+                       patches=("FarField", "FarField") + ("SlipWall",) * 4)
+
+    @classmethod
+    def from_arrays(cls, nodes, cells, patch_faces, patch_bc):
+        """nodes [n][3] f64, cells [N][4] u32 (VTK order), patch_faces list of [m][3]."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.uint32)
+        pn = np.array([len(p) for p in patch_faces], dtype=np.uint32)
+        pf = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.uint32).reshape(-1, 3)
+                                                  for p in patch_faces]), dtype=np.uint32)
+        pbc = np.array([BC_NAMES[b] if isinstance(b, str) else int(b) for b in patch_bc], dtype=np.int32)
+        d = A.MeshDesc(len(nodes), _f64(nodes), len(cells), _u32(cells), len(pn), _u32(pn), _u32(pf), _i32(pbc))
+        h = C.c_void_p()
+        _check(A.lib().nf_mesh_from_desc(C.byref(d), C.byref(h)), what="nf_mesh_from_desc")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            A.lib().nf_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def desc(self):
+        return self._desc
+
+    def arrays(self):
+        d = self._desc
+        nodes = np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes, 3)).copy()
+        cells = np.ctypeslib.as_array(d.cells, shape=(d.n_cells, 4)).copy()
+        pn = np.ctypeslib.as_array(d.patch_n_faces, shape=(d.n_patches,)).copy()
+        pf = np.ctypeslib.as_array(d.patch_faces, shape=(int(pn.sum()), 3)).copy()
+        pbc = np.ctypeslib.as_array(d.patch_bc, shape=(d.n_patches,)).copy()
+        return nodes, cells, pn, pf, pbc
+
+    @property
+    def n_cells(self):
+        return self._desc.n_cells
+
+    def fv_grid(self):
+        """FV_Grid of this mesh: dict of numpy arrays in the reference face order."""
+        s = A.FVSizes()
+        _check(A.lib().nf_fv_sizes_of(self._h, C.byref(s)), what="nf_fv_sizes_of")
+        F, NT, P = s.n_total_faces, s.n_total_cells, s.n_patches
+        out = dict(ij=np.zeros((F, 2), np.uint32), S=np.zeros((F, 3)), r_i=np.zeros((F, 3)),
+                   r_j=np.zeros((F, 3)), volume=np.zeros(NT), centroid=np.zeros((NT, 3)),
+                   patch_first=np.zeros(P, np.uint32), patch_nfaces=np.zeros(P, np.uint32),
+                   delta_S=np.zeros((NT, 3)))
+        _check(A.lib().nf_fv_export(self._h, _u32(out["ij"]), _f64(out["S"]), _f64(out["r_i"]),
+                                    _f64(out["r_j"]), _f64(out["volume"]), _f64(out["centroid"]),
+                                    _u32(out["patch_first"]), _u32(out["patch_nfaces"]),
+                                    _f64(out["delta_S"])), what="nf_fv_export")
+        out["sizes"] = (s.n_interior_cells, s.n_total_cells, s.n_interior_faces, s.n_total_faces, P)
+        return out
+
+
+class SolverData:
+    """Read-only view of the solver fields (SolverData accessors); every
+    accessor copies from the device in the reference layout."""
+
+    def __init__(self, solver):
+        self._s = solver
+
+    def _field(self, which, cols):
+        out = np.zeros((self._s.N, cols))
+        _check(A.lib().nf_get_field(self._s._ctx, which, _f64(out)), self._s._ctx, "nf_get_field")
+        return out
+
+    def get_solution(self):
+        out = np.zeros((self._s.N, 5))
+        _check(A.lib().nf_get_state(self._s._ctx, _f64(out)), self._s._ctx, "nf_get_state")
+        return out
+
+    def get_solution_old(self):
+        return self._field(A.NF_FIELD_STATE_OLD, 5)
+
+    def get_primvars(self):
+        return self._field(A.NF_FIELD_PRIM, 5)
+
+    def get_primvars_gradient(self):
+        """[N][15], per cell the 5x3 gradient column-major (offset k + 5 d)."""
+        return self._field(A.NF_FIELD_GRADIENT, 15)
+
+    def get_primvars_limiter(self):
+        return self._field(A.NF_FIELD_LIMITER, 5)
+
+    def get_flux_balance(self):
+        return self._field(A.NF_FIELD_RESIDUAL, 5)
+
+
+class EulerSolver:
+    """Drop-in for EulerSolver(config, primal_grid, FV_grid) on one GPU."""
+
+    def __init__(self, config: Config, mesh: Mesh):
+        self.config = config
+        self.mesh = mesh
+        self._params = config.to_params()
+        ctx = C.c_void_p()
+        rc = A.lib().nf_create(C.byref(mesh.desc), C.byref(self._params), C.byref(ctx))
+        if rc != A.NF_OK:
+            raise RuntimeError(f"nf_create failed with status {rc}")
+        self._ctx = ctx
+        s = A.FVSizes()
+        A.lib().nf_ctx_sizes(self._ctx, C.byref(s))
+        self.sizes = s
+        self.N = s.n_interior_cells
+        self._data = SolverData(self)
+
+    def __del__(self):
+        if getattr(self, "_ctx", None):
+            A.lib().nf_destroy(self._ctx)
+            self._ctx = None
+
+    # --- the reference's public surface ---
+    def calc_timestep(self, config: Config):
+        dt = C.c_double()
+        _check(A.lib().nf_calc_timestep(self._ctx, C.byref(dt)), self._ctx, "nf_calc_timestep")
+        config.delta_time = dt.value
+
+    def step(self, config: Config):
+        _check(A.lib().nf_step(self._ctx, config.delta_time), self._ctx, "nf_step")
+
+    def get_solver_data(self):
+        return self._data
+
+    def get_solver_type(self):
+        return "Euler"
+
+    # --- additions ---
+    def set_state(self, U, U_old=None):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.N, 5)
+        uo = None if U_old is None else _f64(np.ascontiguousarray(U_old, dtype=np.float64))
+        _check(A.lib().nf_set_state(self._ctx, _f64(U), uo), self._ctx, "nf_set_state")
+
+    def set_freestream(self):
+        _check(A.lib().nf_set_freestream(self._ctx), self._ctx, "nf_set_freestream")
+
+    def eval_flux_balance(self):
+        _check(A.lib().nf_eval_flux_balance(self._ctx), self._ctx, "nf_eval_flux_balance")
+
+    def run(self, n_steps, residual_drop=0.0, check_every=10, use_graph=True):
+        o = A.RunOpts(n_steps, residual_drop, check_every, int(use_graph))
+        st = A.RunStats()
+        _check(A.lib().nf_run(self._ctx, C.byref(o), C.byref(st)), self._ctx, "nf_run")
+        return {f: getattr(st, f) for f, _ in A.RunStats._fields_}
+
+    def residuals(self, max_n=1 << 20):
+        out = np.zeros(max_n)
+        n = C.c_int32()
+        _check(A.lib().nf_get_residuals(self._ctx, _f64(out), max_n, C.byref(n)), self._ctx, "nf_get_residuals")
+        return out[: n.value].copy()
+
+    def time_kernels(self, n_steps, max_k=32):
+        arr = (A.KernelTime * max_k)()
+        n = C.c_int32()
+        _check(A.lib().nf_time_kernels(self._ctx, n_steps, arr, max_k, C.byref(n)), self._ctx, "nf_time_kernels")
+        return {arr[i].name.decode(): (arr[i].avg_ms, arr[i].calls) for i in range(n.value)}
+
+
+class Driver:
+    """The reference's time loop (src/Driver.cpp:31-57) without VTK output."""
+
+    def __init__(self, config: Config, mesh: Mesh):
+        self.config = config
+        self.solver = EulerSolver(config, mesh)
+
+    def solve(self):
+        cfg = self.config
+        cfg.timestep = 0
+        cfg.time = 0.0
+        while True:
+            self.solver.calc_timestep(cfg)
+            self.solver.step(cfg)
+            cfg.timestep += 1
+            cfg.time += cfg.delta_time
+            if cfg.timestep >= cfg.n_timesteps:
+                break
