@@ -70,7 +70,20 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=48)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -82,7 +95,7 @@ def main():
     t0 = time.perf_counter()
     mesh = nf.Mesh.box(args.n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
     U0 = pulse_state(mesh)
-    cfg = pulse_config(mach=0.5)
+    cfg = pulse_config(mach=0.5, cell_order=args.order)
     solver = nf.EulerSolver(cfg, mesh)
     N = solver.N
     F = solver.sizes.n_total_faces
@@ -139,7 +152,7 @@ def main():
         "data": "synthetic (generated jittered Kuhn mesh, Gaussian pulse)",
         "config": {"workload": f"plain FP64 TVD-RK3 step, unit cube {args.n}^3 hex->6 tets "
                                f"({N} cells), jitter 0.15, Mach 0.5 + pulse, farfield, CFL 0.5",
-                   "cells": N, "faces": F, "graph_steps": args.graph_steps},
+                   "cells": N, "faces": F, "graph_steps": args.graph_steps, "cell_order": args.order},
         "step_roofline_frac": round(step_frac, 4),
         "kernels": kern,
         "roofline": roofline,

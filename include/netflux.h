@@ -42,7 +42,7 @@ enum { NF_BC_NOSLIP_WALL = 0, NF_BC_SLIP_WALL = 1, NF_BC_FARFIELD = 2 }; /* Boun
 enum { NF_PREC_FP64 = 0,   /* everything FP64, bit-identical op order       */
        NF_PREC_MIXED = 1   /* FP32 flux/gradient/limiter, FP64 state/RK/norms */ };
 /* cell numbering used on the device */
-enum { NF_ORDER_INPUT = 0, NF_ORDER_RCM = 1 };
+enum { NF_ORDER_INPUT = 0, NF_ORDER_RCM = 1, NF_ORDER_MORTON = 2 };
 
 /* ---- primal mesh: what PrimalGrid holds after read_su2_mesh ------------
  * (src/geometry/PrimalGrid.cpp:140-244, after salome_to_vtk_connectivity).

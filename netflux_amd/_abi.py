@@ -16,7 +16,7 @@ NF_LIMITER_NONE, NF_LIMITER_BARTH = 0, 1
 NF_FLUX_RUSANOV, NF_FLUX_HLLC = 0, 1
 NF_BC_NOSLIP_WALL, NF_BC_SLIP_WALL, NF_BC_FARFIELD = 0, 1, 2
 NF_PREC_FP64, NF_PREC_MIXED = 0, 1
-NF_ORDER_INPUT, NF_ORDER_RCM = 0, 1
+NF_ORDER_INPUT, NF_ORDER_RCM, NF_ORDER_MORTON = 0, 1, 2
 NF_FIELD_PRIM, NF_FIELD_LIMITER, NF_FIELD_GRADIENT, NF_FIELD_RESIDUAL, NF_FIELD_STATE_OLD = range(5)
 
 u32p = C.POINTER(C.c_uint32)

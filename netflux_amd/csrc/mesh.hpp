@@ -39,17 +39,17 @@ struct FVGrid {
     std::vector<double> dS;           // [NT][3] interior-face half |S| sums
 };
 
-// Device layout, one record per device cell (structure of arrays).
+// Device layout, one record per device cell.
 // Slot s of a cell holds its s-th face in increasing reference face index,
 // so gathers add contributions in the reference's scatter order.
 struct CellLayout {
     uint32_t N = 0;
     std::vector<uint32_t> perm;   // device cell -> reference cell
     std::vector<uint32_t> iperm;  // reference cell -> device cell
-    std::vector<uint32_t> cnodes; // [4][N] tet node ids (VTK order)
-    std::vector<uint32_t> nbr;    // [4][N] neighbour words (see below)
+    std::vector<uint32_t> cnodes; // [N][4] tet node ids (VTK order)
+    std::vector<uint32_t> nbr;    // [N][4] neighbour words (see below)
     std::vector<uint32_t> fcode;  // [N] 4 x 8-bit face codes
-    std::vector<uint32_t> ghost;  // [4][N] reference ghost id (boundary slots)
+    std::vector<uint32_t> ghost;  // [N][4] reference ghost id (boundary slots)
 };
 
 // neighbour word: bit31 = boundary.  interior: bits 29-30 neighbour slot,
@@ -72,10 +72,11 @@ void build_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly,
 // Throws MeshError on inconsistent input.
 void build_fv_grid(const PrimalMesh &m, FVGrid &g);
 
-// order: 0 = input order, 1 = reverse Cuthill-McKee.
+// order: 0 = input order, 1 = reverse Cuthill-McKee, 2 = Morton (Z-order of centroids).
 void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order,
                        CellLayout &L);
 
 std::vector<uint32_t> rcm_order(const FVGrid &g);
+std::vector<uint32_t> morton_order(const FVGrid &g);
 
 }  // namespace nf

@@ -23,6 +23,23 @@ using namespace nf;
 
 namespace {
 
+// First level of the per-block partial reductions: block b reduces the
+// contiguous chunk [b*RCHUNK, (b+1)*RCHUNK) in a fixed order.
+constexpr int RCHUNK = 2048;
+template <bool MIN>
+__global__ __launch_bounds__(BLOCK) void k_reduce(const double *in, int n, double *out, const int *stop) {
+    __shared__ double sh[BLOCK / 64];
+    if (*stop) return;
+    const int base = blockIdx.x * RCHUNK + threadIdx.x * (RCHUNK / BLOCK);
+    double v = MIN ? DBL_MAX : 0.0;
+    for (int q = 0; q < RCHUNK / BLOCK; ++q) {
+        const int i = base + q;
+        if (i < n) v = MIN ? ((in[i] < v) ? in[i] : v) : v + in[i];
+    }
+    v = block_reduce<MIN>(v, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
 __global__ void k_finalize_dt(const double *part, int nb, double *dt, double *time_acc,
                               int *nstep, int *err, int *stop, int advance) {
     __shared__ double sh[BLOCK / 64];
@@ -80,9 +97,9 @@ __global__ void k_prim(const double *U, double *V, uint32_t N) {
     const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= N) return;
     double u[5], v[5];
-    for (int k = 0; k < 5; ++k) u[k] = U[(size_t)k * N + c];
+    for (int k = 0; k < 5; ++k) u[k] = U[5 * (size_t)c + k];
     cons_to_prim(u, v);
-    for (int k = 0; k < 5; ++k) V[(size_t)k * N + c] = v[k];
+    for (int k = 0; k < 5; ++k) V[5 * (size_t)c + k] = v[k];
 }
 
 struct DevBuf {
@@ -108,7 +125,8 @@ struct nf_ctx {
     uint32_t *cnodes = nullptr, *nbr = nullptr, *fcode = nullptr;
     double *Ua = nullptr, *Ub = nullptr, *V = nullptr, *FV = nullptr;
     double *grad_aux = nullptr, *psi_aux = nullptr, *R_aux = nullptr;
-    double *dt_part = nullptr, *res_part = nullptr;
+    double *dt_part = nullptr, *res_part = nullptr, *red2 = nullptr;
+    int nb2 = 0;  // second-level partials
     double *scal = nullptr;  // [0] dt, [1] time
     int *iscal = nullptr;    // [0] nstep [1] err [2] stop [3] converged [4] bad_flux [5] bad_state
     double *hist = nullptr;
@@ -151,7 +169,7 @@ int dalloc(nf_ctx *ctx, T **p, size_t n) {
     return NF_OK;
 }
 
-DevMesh devmesh(const nf_ctx *c) { return DevMesh{c->N, c->nodes, c->cnodes, c->nbr, c->fcode}; }
+DevMesh devmesh(const nf_ctx *c) { return DevMesh{c->N, 3.0, 6.0, c->nodes, c->cnodes, c->nbr, c->fcode}; }
 
 // kernel launch with optional per-kernel event timing
 template <typename K, typename... Args>
@@ -243,8 +261,11 @@ int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
     int *is = ctx->iscal;
     if ((rc = run_sweepA(ctx, !rk3, false))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, rk3 ? ctx->Ub : ctx->Ua, true, false))) return rc;
-    if ((rc = launch(ctx, "finalize_res", k_finalize_res, dim3(1), dim3(BLOCK), (const double *)ctx->res_part,
-                     ctx->nb, ctx->hist, ctx->hist_cap, (const int *)is, is + 2, drop, is + 3,
+    if ((rc = launch(ctx, "reduce_res", k_reduce<false>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->res_part,
+                     ctx->nb, ctx->red2 + ctx->nb2, (const int *)(is + 2))))
+        return rc;
+    if ((rc = launch(ctx, "finalize_res", k_finalize_res, dim3(1), dim3(BLOCK), (const double *)(ctx->red2 + ctx->nb2),
+                     ctx->nb2, ctx->hist, ctx->hist_cap, (const int *)is, is + 2, drop, is + 3,
                      (int)check_now)))
         return rc;
     if (rk3) {
@@ -254,8 +275,11 @@ int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
         if ((rc = run_sweepA(ctx, true, false))) return rc;
         if ((rc = run_sweepB(ctx, 3, ctx->Ub, Uo, ctx->Ua, false, false))) return rc;
     }
-    if ((rc = launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->dt_part,
-                     ctx->nb, ctx->scal, ctx->scal + 1, is, is + 1, is + 2, 1)))
+    if ((rc = launch(ctx, "reduce_dt", k_reduce<true>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->dt_part,
+                     ctx->nb, ctx->red2, (const int *)(is + 2))))
+        return rc;
+    if ((rc = launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->red2,
+                     ctx->nb2, ctx->scal, ctx->scal + 1, is, is + 1, is + 2, 1)))
         return rc;
     if (drop > 0.0)
         if ((rc = launch(ctx, "stop_if_converged", k_stop_if_converged, dim3(1), dim3(1),
@@ -268,8 +292,11 @@ int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
 int enqueue_timestep(nf_ctx *ctx) {
     int rc;
     if ((rc = run_sweepA(ctx, true, false))) return rc;
-    return launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->dt_part,
-                  ctx->nb, ctx->scal, ctx->scal + 1, ctx->iscal, ctx->iscal + 1, ctx->iscal + 2, 0);
+    if ((rc = launch(ctx, "reduce_dt", k_reduce<true>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->dt_part,
+                     ctx->nb, ctx->red2, (const int *)(ctx->iscal + 2))))
+        return rc;
+    return launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->red2,
+                  ctx->nb2, ctx->scal, ctx->scal + 1, ctx->iscal, ctx->iscal + 1, ctx->iscal + 2, 0);
 }
 
 int check_errors(nf_ctx *ctx) {
@@ -320,26 +347,22 @@ int ensure_hist(nf_ctx *ctx, int need) {
     return NF_OK;
 }
 
-int upload_state_soa(nf_ctx *ctx, const double *aos, double *dev) {
+// reference-order AoS [N][rows] <-> device-order AoS [N][rows]
+int upload_aos(nf_ctx *ctx, const double *aos, double *dev) {
     const uint32_t N = ctx->N;
-    std::vector<double> soa((size_t)5 * N);
-    for (uint32_t d = 0; d < N; ++d) {
-        const double *u = aos + 5 * (size_t)ctx->perm[d];
-        for (int k = 0; k < 5; ++k) soa[(size_t)k * N + d] = u[k];
-    }
-    HIPC(hipMemcpy(dev, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> d((size_t)5 * N);
+    for (uint32_t c = 0; c < N; ++c) std::memcpy(&d[5 * (size_t)c], aos + 5 * (size_t)ctx->perm[c], 40);
+    HIPC(hipMemcpy(dev, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
     return NF_OK;
 }
 
-int download_soa(nf_ctx *ctx, const double *dev, int rows, double *aos) {
+int download_aos(nf_ctx *ctx, const double *dev, int rows, double *aos) {
     const uint32_t N = ctx->N;
-    std::vector<double> soa((size_t)rows * N);
+    std::vector<double> d((size_t)rows * N);
     HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(soa.data(), dev, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t d = 0; d < N; ++d) {
-        double *u = aos + (size_t)rows * ctx->perm[d];
-        for (int k = 0; k < rows; ++k) u[k] = soa[(size_t)k * N + d];
-    }
+    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < N; ++c)
+        std::memcpy(aos + (size_t)rows * ctx->perm[c], &d[(size_t)rows * c], rows * sizeof(double));
     return NF_OK;
 }
 
@@ -396,7 +419,7 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     ctx->perm = L.perm;
     ctx->iperm = L.iperm;
     ctx->fs = make_farfield<double>(prm->primvars_inf);
-    ctx->nb = (int)((g.N + BLOCK - 1) / BLOCK);
+    ctx->nb = (int)((g.N + CELLS - 1) / CELLS);
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const uint32_t N = g.N;
     int rc;
@@ -410,6 +433,8 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     if ((rc = dalloc(ctx, &ctx->FV, 20 * (size_t)N))) return rc;
     if ((rc = dalloc(ctx, &ctx->dt_part, (size_t)ctx->nb))) return rc;
     if ((rc = dalloc(ctx, &ctx->res_part, (size_t)ctx->nb))) return rc;
+    ctx->nb2 = (ctx->nb + RCHUNK - 1) / RCHUNK;
+    if ((rc = dalloc(ctx, &ctx->red2, 2 * (size_t)ctx->nb2))) return rc;
     if ((rc = dalloc(ctx, &ctx->scal, 4))) return rc;
     if ((rc = dalloc(ctx, &ctx->iscal, 8))) return rc;
     {
@@ -453,20 +478,20 @@ int nf_set_freestream(nf_ctx *ctx) {
     std::vector<double> aos((size_t)5 * ctx->N);
     for (uint32_t c = 0; c < ctx->N; ++c)
         for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = U[k];
-    int rc = upload_state_soa(ctx, aos.data(), ctx->Ua);
+    int rc = upload_aos(ctx, aos.data(), ctx->Ua);
     if (rc) return rc;
     if ((rc = after_state_set(ctx))) return rc;
     // primvars = V_inf exactly (set_freestream_values, src/SolverData.cpp:50-58)
     for (uint32_t c = 0; c < ctx->N; ++c)
         for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = V[k];
-    if ((rc = upload_state_soa(ctx, aos.data(), ctx->V))) return rc;
+    if ((rc = upload_aos(ctx, aos.data(), ctx->V))) return rc;
     ctx->uold_zero = true;
     return NF_OK;
 }
 
 int nf_set_state(nf_ctx *ctx, const double *U, const double *U_old) {
     if (!ctx || !U) return NF_E_ARG;
-    int rc = upload_state_soa(ctx, U, ctx->Ua);
+    int rc = upload_aos(ctx, U, ctx->Ua);
     if (rc) return rc;
     if (U_old) {
         // U_old is held in Ua between steps: a non-zero U_old different from U
@@ -628,7 +653,7 @@ int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
 
 int nf_get_state(nf_ctx *ctx, double *U) {
     if (!ctx || !U) return NF_E_ARG;
-    return download_soa(ctx, ctx->Ua, 5, U);
+    return download_aos(ctx, ctx->Ua, 5, U);
 }
 
 // Evaluates the flux balance of the current state once (Solver::
@@ -645,7 +670,7 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
     }
     if ((rc = clear_flags(ctx))) return rc;
     // primvars of the current state
-    hipLaunchKernelGGL(k_prim, dim3(ctx->nb), dim3(BLOCK), 0, ctx->stream, ctx->Ua, ctx->V, N);
+    hipLaunchKernelGGL(k_prim, dim3((N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua, ctx->V, N);
     HIPC(hipGetLastError());
     if ((rc = run_sweepA(ctx, false, true))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, ctx->Ub, false, true))) return rc;
@@ -657,21 +682,21 @@ int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
     if (!ctx || !out) return NF_E_ARG;
     switch (which) {
     case NF_FIELD_PRIM:
-        return download_soa(ctx, ctx->V, 5, out);
+        return download_aos(ctx, ctx->V, 5, out);
     case NF_FIELD_STATE_OLD: {
         if (ctx->uold_zero) {
             std::memset(out, 0, sizeof(double) * 5 * ctx->N);
             return NF_OK;
         }
-        return download_soa(ctx, ctx->Ua, 5, out);
+        return download_aos(ctx, ctx->Ua, 5, out);
     }
     case NF_FIELD_LIMITER:
     case NF_FIELD_GRADIENT:
     case NF_FIELD_RESIDUAL:
         if (!ctx->aux_valid) return fail(ctx, NF_E_ARG, "call nf_eval_flux_balance first");
-        if (which == NF_FIELD_LIMITER) return download_soa(ctx, ctx->psi_aux, 5, out);
-        if (which == NF_FIELD_GRADIENT) return download_soa(ctx, ctx->grad_aux, 15, out);
-        return download_soa(ctx, ctx->R_aux, 5, out);
+        if (which == NF_FIELD_LIMITER) return download_aos(ctx, ctx->psi_aux, 5, out);
+        if (which == NF_FIELD_GRADIENT) return download_aos(ctx, ctx->grad_aux, 15, out);
+        return download_aos(ctx, ctx->R_aux, 5, out);
     default:
         return fail(ctx, NF_E_ARG, "unknown field");
     }

@@ -61,7 +61,7 @@ class Config:
     # reference quirk: U_old is zero on the first TVD-RK3 step
     rk3_zero_old_first_step: bool = True
     precision: str = "fp64"               # fp64 | mixed
-    cell_order: str = "input"             # input | rcm
+    cell_order: str = "morton"            # input | rcm | morton (device cell numbering)
     # run-time state held by Config in the reference
     delta_time: float = 0.0
     time: float = 0.0
@@ -82,7 +82,8 @@ class Config:
         p.check_physical_validity = int(self.check_physical_validity)
         p.rk3_zero_old_first_step = int(self.rk3_zero_old_first_step)
         p.precision = {"fp64": A.NF_PREC_FP64, "mixed": A.NF_PREC_MIXED}[self.precision]
-        p.cell_order = {"input": A.NF_ORDER_INPUT, "rcm": A.NF_ORDER_RCM}[self.cell_order]
+        p.cell_order = {"input": A.NF_ORDER_INPUT, "rcm": A.NF_ORDER_RCM,
+                        "morton": A.NF_ORDER_MORTON}[self.cell_order]
         return p
 
 
