@@ -1,24 +1,29 @@
 // FP64 stage kernels of the explicit Euler step.
 //
-// Mapping: one tetrahedron per lane quad, one face per lane (lane = 4*cell +
-// slot).  Slot s of a cell is its s-th face in increasing reference face
-// index, and every per-cell accumulation is formed by adding the four lane
-// contributions of the quad in slot order, ((((0 + t0) + t1) + t2) + t3),
-// with DPP quad permutes -- the same terms in the same order as the
-// reference's face-ordered scatter loops, so FP64 results are bit-identical
-// to a serial evaluation, with no atomics and no colouring.  Geometry
-// (volume, centroid, face normal, centroid-to-face vector) is recomputed from
-// the four nodes with the reference formulas instead of streamed.
+// Mapping: one tetrahedron per lane.  The lane visits its four faces in slot
+// order -- slot s is the cell's s-th face in increasing reference face index
+// -- so every per-cell accumulation (Green-Gauss gradient, flux balance,
+// Delta S) adds the same terms in the same order as the reference's
+// face-ordered scatter loops, and FP64 results are identical to a serial
+// evaluation of the reference formulas, with no atomics and no colouring.
+// Geometry (volume, centroid, face normals, centroid-to-face vectors) is
+// recomputed from the four nodes with the reference formulas instead of
+// being streamed from memory.
 //
 //   sweep A (per stage): V -> Green-Gauss gradient, neighbour min/max, Barth
-//            limiter, limited face value of the lane's face
+//            limiter, limited face values of the four faces
 //            (+ Delta S and the CFL time step when requested).
-//   sweep B (per stage): face values -> Rusanov flux of the lane's face
-//            (wall/farfield ghost from the reconstructed state), flux balance,
-//            RK stage combination, prim(U) of the new state.
+//   sweep B (per stage): face values -> Rusanov fluxes of the four faces,
+//            flux balance, RK stage combination, prim(U) of the new state.
+//   Boundary faces read their ghost state from small per-boundary-face
+//   buffers filled by k_ghost<false> (ghost of the cell-centre state, for the
+//   gradient and min/max) and k_ghost<true> (ghost of the reconstructed face
+//   state, for the boundary flux), so the sweeps carry no boundary code.
 //
-// Layouts (device cell order): V, U: [N][5]; face values FV: [N][4][5];
-// cnodes, nbr: [N][4]; fcode: [N]; nodes: [n_nodes][4] (x, y, z, pad).
+// Layouts: cell fields are tiled by 64 cells (one wavefront):
+//   V, U  : [tile][5][64]          FV : [tile][4 slots][5][64]
+// so a wave's own-cell loads and stores are fully coalesced; cnodes and nbr
+// are [N][4] u32, fcode [N], nodes [n_nodes][4] (x, y, z, pad).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -28,8 +33,15 @@
 
 namespace nf {
 
-constexpr int BLOCK = 256;            // threads
-constexpr int CELLS = BLOCK / 4;      // cells per block
+constexpr int BLOCK = 256;  // threads = cells per block
+constexpr int TILE = 64;    // cells per layout tile
+
+__device__ __forceinline__ size_t tix(uint32_t c, int k) {  // [tile][5][64]
+    return ((size_t)(c >> 6) * 5 + k) * TILE + (c & 63u);
+}
+__device__ __forceinline__ size_t fix(uint32_t c, uint32_t s, int k) {  // [tile][4][5][64]
+    return (((size_t)(c >> 6) * 4 + s) * 5 + k) * TILE + (c & 63u);
+}
 
 struct DevMesh {
     uint32_t N;
@@ -57,43 +69,9 @@ __device__ __forceinline__ double dsel(uint32_t q, double a0, double a1, double 
     r = (q == 3u) ? a3 : r;
     return r;
 }
-__device__ __forceinline__ V3d vsel(uint32_t q, const V3d &p0, const V3d &p1, const V3d &p2, const V3d &p3) {
-    return {dsel(q, p0.x, p1.x, p2.x, p3.x), dsel(q, p0.y, p1.y, p2.y, p3.y), dsel(q, p0.z, p1.z, p2.z, p3.z)};
-}
 
-// value held by lane Q of this lane's quad (DPP quad_perm [Q,Q,Q,Q])
-template <int Q>
-__device__ __forceinline__ double qget(double v) {
-    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ V3d qget3(V3d v, int q) {
-    switch (q) {
-    case 0: return {qget<0>(v.x), qget<0>(v.y), qget<0>(v.z)};
-    case 1: return {qget<1>(v.x), qget<1>(v.y), qget<1>(v.z)};
-    case 2: return {qget<2>(v.x), qget<2>(v.y), qget<2>(v.z)};
-    default: return {qget<3>(v.x), qget<3>(v.y), qget<3>(v.z)};
-    }
-}
-// ((((0 + x0) + x1) + x2) + x3) over the quad, identical in all four lanes
-__device__ __forceinline__ double qsum(double x) {
-    const double a = qget<0>(x), b = qget<1>(x), c = qget<2>(x), d = qget<3>(x);
-    return (((0.0 + a) + b) + c) + d;
-}
-__device__ __forceinline__ double qmax(double init, double x) {
-    const double a = qget<0>(x), b = qget<1>(x), c = qget<2>(x), d = qget<3>(x);
-    return smax(smax(smax(smax(init, a), b), c), d);
-}
-__device__ __forceinline__ double qmin(double init, double x) {
-    const double a = qget<0>(x), b = qget<1>(x), c = qget<2>(x), d = qget<3>(x);
-    return smin(smin(smin(smin(init, a), b), c), d);
-}
-
-// IEEE maxNum/minNum of two doubles (v_max_f64 / v_min_f64).  Over finite
-// values the maximum/minimum of a set does not depend on the order of
-// evaluation, so the quad reductions below may use a butterfly; only the sign
+// IEEE maxNum/minNum (v_max_f64 / v_min_f64): over finite values the
+// maximum/minimum of a set does not depend on evaluation order; only the sign
 // of a zero result can differ from a sequential std::max / std::min.
 __device__ __forceinline__ double dmax(double a, double b) {
     double r;
@@ -104,21 +82,6 @@ __device__ __forceinline__ double dmin(double a, double b) {
     double r;
     asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
-}
-template <int CTRL>
-__device__ __forceinline__ double qperm(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-// quad_perm [1,0,3,2] then [2,3,0,1]
-__device__ __forceinline__ double qmax_all(double x) {
-    x = dmax(x, qperm<0xB1>(x));
-    return dmax(x, qperm<0x4E>(x));
-}
-__device__ __forceinline__ double qmin_all(double x) {
-    x = dmin(x, qperm<0xB1>(x));
-    return dmin(x, qperm<0x4E>(x));
 }
 
 // Division by a divisor used several times.  The f64 divide of the target is
@@ -156,11 +119,11 @@ __device__ __forceinline__ void cons_to_prim_d(const double *U, double *V) {
 
 // NumericalFlux::rusanov (src/Numerics.cpp:10-18) with the EulerEqs helpers
 // inlined and one reciprocal per repeated divisor (|S|, rho_L, rho_R)
-__device__ __forceinline__ void rusanov_d(const double *UL, const double *UR, Vec3T<double> S, double *F) {
+__device__ __forceinline__ void rusanov_d(const double *UL, const double *UR, V3d S, double *F) {
     const double z = (S.x * S.x + S.y * S.y) + S.z * S.z;
     const double area = sqrt(z);
     const Rcp ia = rcp_of(area);
-    Vec3T<double> n = S;
+    V3d n = S;
     if (z > 0.0) n = {rdiv(S.x, ia), rdiv(S.y, ia), rdiv(S.z, ia)};
     const Rcp iL = rcp_of(UL[0]), iR = rcp_of(UR[0]);
     const double G = Consts<double>::GAMMA, GM1 = Consts<double>::GM1;
@@ -181,13 +144,13 @@ __device__ __forceinline__ void rusanov_d(const double *UL, const double *UR, Ve
     for (int k = 0; k < 5; ++k) F[k] = h * ((FR[k] + FL[k]) - sm * (UR[k] - UL[k]));
 }
 
-// BC_SlipWall / BC_NoSlipWall (src/Numerics.cpp:59-81); farfield via physics.hpp
-__device__ __forceinline__ void ghost_d(int bc, const double *Vd, Vec3T<double> S, const FarField<double> &fs,
-                                        double *Vg) {
+// BoundaryCondition::calc_ghost_val (src/Numerics.cpp:59-171); slip wall with
+// one reciprocal of |S|, the others as in physics.hpp
+__device__ __forceinline__ void ghost_d(int bc, const double *Vd, V3d S, const FarField<double> &fs, double *Vg) {
     if (bc == BC_SLIP) {
         const double z = (S.x * S.x + S.y * S.y) + S.z * S.z;
         const Rcp ia = rcp_of(sqrt(z));
-        Vec3T<double> n = S;
+        V3d n = S;
         if (z > 0.0) n = {rdiv(S.x, ia), rdiv(S.y, ia), rdiv(S.z, ia)};
         const double vn = (Vd[1] * n.x + Vd[2] * n.y) + Vd[3] * n.z;
         Vg[0] = Vd[0];
@@ -204,9 +167,9 @@ __device__ __forceinline__ void ghost_d(int bc, const double *Vd, Vec3T<double> 
 // minNum, which also returns 1 for a NaN quotient like std::min(1.0, x)
 __device__ __forceinline__ double barth_phi_d(double delta, double vmax, double vmin, double v) {
     const double d2 = double(isign(delta)) * (fabs(delta) + DBL_EPSILON);
-    if (d2 > 0.0) return dmin(1.0, (vmax - v) / d2);
-    if (d2 < 0.0) return dmin(1.0, (vmin - v) / d2);
-    return 1.0;
+    const double num = (d2 > 0.0) ? vmax - v : vmin - v;  // one divide for both branches
+    const double q = num / d2;
+    return (d2 != 0.0) ? dmin(1.0, q) : 1.0;
 }
 
 // Logical block index: blocks are dispatched round robin over the 8 XCDs;
@@ -218,23 +181,23 @@ __device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t nb, int x
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-struct LaneGeo {
+struct CellGeo {
     V3d p0, p1, p2, p3;
     V3d cen;
     double vol;
     Rcp ivol;
 };
 
-// Each lane loads node `slot` of its cell; the quad shares the four nodes.
-__device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, uint32_t s, LaneGeo &g) {
-    const uint32_t n = m.cnodes[4 * (size_t)c + s];
-    const double2 a = *reinterpret_cast<const double2 *>(m.nodes + 4 * (size_t)n);
-    const double b = m.nodes[4 * (size_t)n + 2];
-    const V3d mine = {a.x, a.y, b};
-    g.p0 = qget3(mine, 0);
-    g.p1 = qget3(mine, 1);
-    g.p2 = qget3(mine, 2);
-    g.p3 = qget3(mine, 3);
+__device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo &g) {
+    const uint4 q = *reinterpret_cast<const uint4 *>(m.cnodes + 4 * (size_t)c);
+    auto node = [&](uint32_t n) {
+        const double2 a = *reinterpret_cast<const double2 *>(m.nodes + 4 * (size_t)n);
+        return V3d{a.x, a.y, m.nodes[4 * (size_t)n + 2]};
+    };
+    g.p0 = node(q.x);
+    g.p1 = node(q.y);
+    g.p2 = node(q.z);
+    g.p3 = node(q.w);
     // tetrahedron_calc_volume / _centroid (src/geometry/GeometryUtilities.cpp:289-316)
     const V3d ab = vsub(g.p1, g.p0), ac = vsub(g.p2, g.p0), ad = vsub(g.p3, g.p0);
     g.vol = rdiv(dot3(vcross(ab, ac), ad), rcp_of(m.six));
@@ -242,18 +205,26 @@ __device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, uint32_t
     g.ivol = rcp_of(g.vol);
 }
 
-// Face normal oriented i->j and face centroid (triangle formulas of
-// src/geometry/GeometryUtilities.cpp:318-329, flip of src/geometry/
-// FV_Grid.cpp:247-250) from the face element's node order.
-__device__ __forceinline__ void face_geo(const DevMesh &m, const LaneGeo &g, uint32_t byte, V3d &S, V3d &fc) {
-    const V3d a = vsel(byte & 3u, g.p0, g.p1, g.p2, g.p3);
-    const V3d b = vsel((byte >> 2) & 3u, g.p0, g.p1, g.p2, g.p3);
-    const V3d c = vsel((byte >> 4) & 3u, g.p0, g.p1, g.p2, g.p3);
+// Face normal oriented i->j (triangle formula of src/geometry/
+// GeometryUtilities.cpp:318-324, flip of src/geometry/FV_Grid.cpp:247-250)
+// and centroid-to-face vector from the face element's node order.
+__device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, uint32_t byte, V3d &S, V3d *r) {
+    const uint32_t ia = byte & 3u, ib = (byte >> 2) & 3u, ic = (byte >> 4) & 3u;
+    const V3d a = {dsel(ia, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ia, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
+                   dsel(ia, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
+    const V3d b = {dsel(ib, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ib, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
+                   dsel(ib, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
+    const V3d c = {dsel(ic, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ic, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
+                   dsel(ic, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
     S = vscl(0.5, vcross(vsub(b, a), vsub(c, a)));
     if (byte & 0x40u) S = vscl(-1, S);
-    const V3d s = vadd(vadd(a, b), c);
-    const Rcp i3 = rcp_of(m.three);
-    fc = {rdiv(s.x, i3), rdiv(s.y, i3), rdiv(s.z, i3)};
+    if (r) {
+        // triangle_calc_centroid (GeometryUtilities.cpp:326-329)
+        const V3d s = vadd(vadd(a, b), c);
+        const Rcp i3 = rcp_of(m.three);
+        const V3d fc = {rdiv(s.x, i3), rdiv(s.y, i3), rdiv(s.z, i3)};
+        *r = vsub(fc, g.cen);
+    }
 }
 
 // wave + block min / sum over all threads; result valid in thread 0
@@ -274,12 +245,46 @@ __device__ __forceinline__ double block_reduce(double v, double *sh) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// boundary ghost states.  bface[b] = device cell | slot << 29, bbc[b] = type.
+struct GhostArgs {
+    const uint32_t *bface;
+    const uint8_t *bbc;
+    uint32_t nb;         // boundary faces
+    const double *src;   // V [tile][5][64] (cell) or FV [tile][4][5][64] (face)
+    double *out;         // [nb][5]
+    FarField<double> fs;
+    const int *stop;
+};
+
+// FACE = false: ghost of the cell-centre state (set_constant_ghost_values,
+// src/Solver.cpp:300-330); FACE = true: ghost of the reconstructed face state
+// (src/Solver.cpp:210-212)
+template <bool FACE>
+__global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
+    if (*a.stop) return;
+    const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= a.nb) return;
+    const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
+    CellGeo g;
+    load_cell(m, c, g);
+    V3d S;
+    face_geo(m, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    double Vd[5], Vg[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
+    ghost_d((int)a.bbc[b], Vd, S, a.fs, Vg);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+}
+
+// ---------------------------------------------------------------------------
 struct SweepAArgs {
-    const double *V;   // [N][5]
-    double *FV;        // [N][4][5]
+    const double *V;   // [tile][5][64]
+    const double *Gc;  // [nb][5] ghost of the cell-centre state
+    double *FV;        // [tile][4][5][64]
     double *dt_part;   // per-block CFL dt minimum, or null
     double CFL;
-    FarField<double> fs;
     const int *stop;
     int xcd_remap;
     double *grad_aux;  // [N][15] column-major 5x3 per cell, or null
@@ -292,66 +297,74 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[BLOCK / 64];
     if (*a.stop) return;
     const uint32_t N = m.N;
-    const uint32_t s = threadIdx.x & 3u;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * CELLS + (threadIdx.x >> 2);
+    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
     const bool live = c0 < N;
-    const uint32_t c = live ? c0 : N - 1;  // dead quads mirror the last cell, store nothing
+    const uint32_t c = live ? c0 : N - 1;  // dead lanes mirror the last cell, store nothing
 
+    CellGeo g;
+    load_cell(m, c, g);
+    const uint32_t code = m.fcode[c];
+    const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
+    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    V3d S[4], r[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) face_geo(m, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     double Vc[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) Vc[k] = a.V[5 * (size_t)c + k];
-    LaneGeo g;
-    load_cell(m, c, s, g);
-    const uint32_t byte = (m.fcode[c] >> (8 * s)) & 0xFFu;
-    const bool owner = byte & 0x80u;
-    V3d S, fc;
-    face_geo(m, g, byte, S, fc);
-    const V3d r = vsub(fc, g.cen);
-    const uint32_t w = m.nbr[4 * (size_t)c + s];
-    const bool bnd = w & 0x80000000u;
-    double dtc = DBL_MAX;
+    for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    __builtin_amdgcn_sched_barrier(0);
 
     if (ORDER == 0) {
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) a.FV[(4 * (size_t)c + s) * 5 + k] = Vc[k];
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a.FV[fix(c, s, k)] = Vc[k];
         }
     } else {
-        double Vn[5];
-        if (bnd) {
-            ghost_d((int)(w & 0xFFu), Vc, S, a.fs, Vn);
-        } else {
-            const uint32_t n = w & 0x1FFFFFFFu;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) Vn[k] = a.V[5 * (size_t)n + k];
-        }
         // One variable at a time: gradient row k, its neighbour min/max, the
-        // Barth factor and the face value only depend on variable k, so the
-        // five passes are kept apart (sched_barrier) to bound register use.
+        // Barth factor and the face values depend on variable k only.
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            // Green-Gauss (include/Numerics.hpp:112-127): i/j as in the
-            // reference; cell i adds tmp/vol, cell j subtracts it
-            // (x - t == x + (-t) exactly)
-            const double vi = owner ? Vc[k] : Vn[k], vj = owner ? Vn[k] : Vc[k];
-            const double uf = 0.5 * (vi + vj);
-            const double t0 = rdiv(uf * S.x, g.ivol), t1 = rdiv(uf * S.y, g.ivol), t2 = rdiv(uf * S.z, g.ivol);
-            const double g0 = qsum(owner ? t0 : -t0);
-            const double g1 = qsum(owner ? t1 : -t1);
-            const double g2 = qsum(owner ? t2 : -t2);
-            const double delta = (g0 * r.x + g1 * r.y) + g2 * r.z;
-            double psi = 1.0;
-            if (ORDER == 2) {
-                // neighbour min/max (include/Numerics.hpp:276-291) and the Barth
-                // limiter (include/Numerics.hpp:179-236), faces in order
-                const double vmax = dmax(-DBL_MAX, qmax_all(dmax(vi, vj)));
-                const double vmin = dmin(DBL_MAX, qmin_all(dmin(vi, vj)));
-                psi = dmin(DBL_MAX, qmin_all(barth_phi_d(delta, vmax, vmin, Vc[k])));
+            double Vn[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t idx = w[s] & 0x1FFFFFFFu;
+                Vn[s] = (w[s] & 0x80000000u) ? a.Gc[5 * (size_t)idx + k] : a.V[tix(idx, k)];
+            }
+            // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
+            // cell j subtracts it, faces in reference order
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            double vmax = -DBL_MAX, vmin = DBL_MAX;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const bool owner = (code >> (8 * s + 7)) & 1u;
+                const double vi = owner ? Vc[k] : Vn[s], vj = owner ? Vn[s] : Vc[k];
+                const double uf = 0.5 * (vi + vj);
+                const double t0 = rdiv(uf * S[s].x, g.ivol), t1 = rdiv(uf * S[s].y, g.ivol),
+                             t2 = rdiv(uf * S[s].z, g.ivol);
+                g0 = owner ? g0 + t0 : g0 - t0;
+                g1 = owner ? g1 + t1 : g1 - t1;
+                g2 = owner ? g2 + t2 : g2 - t2;
+                if (ORDER == 2) {
+                    // neighbour min/max (include/Numerics.hpp:276-291)
+                    vmax = dmax(vmax, dmax(vi, vj));
+                    vmin = dmin(vmin, dmin(vi, vj));
+                }
+            }
+            double delta[4];
+            double psi = (ORDER == 2) ? DBL_MAX : 1.0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                delta[s] = (g0 * r[s].x + g1 * r[s].y) + g2 * r[s].z;
+                // Barth limiter (include/Numerics.hpp:179-236), faces in order
+                if (ORDER == 2) psi = dmin(psi, barth_phi_d(delta[s], vmax, vmin, Vc[k]));
             }
             if (live) {
                 // limited reconstruction (include/Numerics.hpp:135-145)
-                a.FV[(4 * (size_t)c + s) * 5 + k] = Vc[k] + psi * delta;
-                if (a.grad_aux && s == 0) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) a.FV[fix(c, s, k)] = Vc[k] + psi * delta[s];
+                if (a.grad_aux) {
                     a.grad_aux[15 * (size_t)c + k] = g0;
                     a.grad_aux[15 * (size_t)c + k + 5] = g1;
                     a.grad_aux[15 * (size_t)c + k + 10] = g2;
@@ -364,33 +377,40 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA(DevMesh m, SweepAArgs a) {
     if (a.dt_part) {
         // Delta S over interior faces (src/Solver.cpp:287-297) and the CFL
         // step of the cell (src/Solver.cpp:254-266)
-        const double h0 = bnd ? 0.0 : 0.5 * sabs(S.x), h1 = bnd ? 0.0 : 0.5 * sabs(S.y),
-                     h2 = bnd ? 0.0 : 0.5 * sabs(S.z);
-        const double d0 = qsum(h0), d1 = qsum(h1), d2 = qsum(h2);
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (!(w[s] & 0x80000000u)) {
+                d0 += 0.5 * fabs(S[s].x);
+                d1 += 0.5 * fabs(S[s].y);
+                d2 += 0.5 * fabs(S[s].z);
+            }
+        }
         const double cs = sound_speed_prim(Vc);
-        const double sx = (sabs(Vc[1]) + cs) * d0;
-        const double sy = (sabs(Vc[2]) + cs) * d1;
-        const double sz = (sabs(Vc[3]) + cs) * d2;
+        const double sx = (fabs(Vc[1]) + cs) * d0;
+        const double sy = (fabs(Vc[2]) + cs) * d1;
+        const double sz = (fabs(Vc[3]) + cs) * d2;
         const double t = a.CFL * g.vol / ((sx + sy) + sz);
-        dtc = (live && t < DBL_MAX) ? t : DBL_MAX;  // NaN/+inf never win std::min
+        const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;  // NaN/+inf never win std::min
         const double v = block_reduce<true>(dtc, sh);
         if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
     }
 }
 
+// ---------------------------------------------------------------------------
 struct SweepBArgs {
-    const double *FV;  // [N][4][5]
-    const double *U;   // [N][5] state entering the stage
-    const double *Uo;  // [N][5] U_old, or null for zero
-    double *Uout;      // [N][5] (may alias U)
-    double *Vout;      // [N][5] prim(Uout)
+    const double *FV;  // [tile][4][5][64]
+    const double *Gf;  // [nb][5] ghost of the reconstructed face state
+    const double *U;   // [tile][5][64] state entering the stage
+    const double *Uo;  // U_old, or null for zero
+    double *Uout;      // (may alias U)
+    double *Vout;      // prim(Uout)
     const double *dt;  // device scalar
     int kind;          // 1: U + dt/vol R, 2: RK3 stage 2, 3: RK3 stage 3
     double *res_part;  // per-block sum of R_rho^2, or null
     int *bad_flux;     // count of cells with non-finite R (if check)
     int check;
     int *bad_state;    // count of cells with rho<=0, p<=0 or NaN in Uout
-    FarField<double> fs;
     const int *stop;
     int xcd_remap;
     double *R_aux;     // [N][5] or null: store R, skip the update
@@ -400,45 +420,52 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB(DevMesh m, SweepBArgs a) {
     __shared__ double sh[BLOCK / 64];
     if (*a.stop) return;
     const uint32_t N = m.N;
-    const uint32_t s = threadIdx.x & 3u;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * CELLS + (threadIdx.x >> 2);
+    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
     const bool live = c0 < N;
     const uint32_t c = live ? c0 : N - 1;
 
-    LaneGeo g;
-    load_cell(m, c, s, g);
-    const uint32_t byte = (m.fcode[c] >> (8 * s)) & 0xFFu;
-    const bool owner = byte & 0x80u;
-    V3d S, fc;
-    face_geo(m, g, byte, S, fc);
-    const uint32_t w = m.nbr[4 * (size_t)c + s];
-    double Vo[5], Vn[5];
+    CellGeo g;
+    load_cell(m, c, g);
+    const uint32_t code = m.fcode[c];
+    const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
+    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 5; ++k) Vo[k] = a.FV[(4 * (size_t)c + s) * 5 + k];
-    if (w & 0x80000000u) {
-        ghost_d((int)(w & 0xFFu), Vo, S, a.fs, Vn);
-    } else {
-        const uint32_t n = w & 0x1FFFFFFFu, sn = (w >> 29) & 3u;
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t byte = (code >> (8 * s)) & 0xFFu;
+        const bool owner = byte & 0x80u;
+        V3d S;
+        face_geo(m, g, byte, S, nullptr);
+        double Vo[5], Vn[5];
+        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
+        const bool bnd = w[s] & 0x80000000u;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) Vn[k] = a.FV[(4 * (size_t)n + sn) * 5 + k];
+        for (int k = 0; k < 5; ++k) {
+            Vo[k] = a.FV[fix(c, s, k)];
+            Vn[k] = bnd ? a.Gf[5 * (size_t)idx + k] : a.FV[fix(idx, sn, k)];
+        }
+        // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
+        double UL[5], UR[5], F[5];
+        if (owner) {
+            prim_to_cons(Vo, UL);
+            prim_to_cons(Vn, UR);
+        } else {
+            prim_to_cons(Vn, UL);
+            prim_to_cons(Vo, UR);
+        }
+        rusanov_d(UL, UR, S, F);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) R[k] = owner ? R[k] - F[k] : R[k] + F[k];  // R_i -= F, R_j += F
+        __builtin_amdgcn_sched_barrier(0);
     }
-    // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
-    double UL[5], UR[5], F[5];
-    prim_to_cons(owner ? Vo : Vn, UL);
-    prim_to_cons(owner ? Vn : Vo, UR);
-    rusanov_d(UL, UR, S, F);
-    double R[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) R[k] = qsum(owner ? -F[k] : F[k]);  // R_i -= F, R_j += F
-
-    if (live && s == 0 && a.check) {
+    if (live && a.check) {
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < 5; ++k) ok = ok && (R[k] - R[k] == 0.0);
         if (!ok) atomicAdd(a.bad_flux, 1);
     }
     if (a.R_aux) {
-        if (live && s == 0) {
+        if (live) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
         }
@@ -449,46 +476,49 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB(DevMesh m, SweepBArgs a) {
         if (a.kind == 1) {
             const double f = rdiv(dt, g.ivol);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Un[k] = a.U[5 * (size_t)c + k] + f * R[k];
+            for (int k = 0; k < 5; ++k) Un[k] = a.U[tix(c, k)] + f * R[k];
         } else if (a.kind == 2) {
             const double f = rdiv(1.0 / 4.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[5 * (size_t)c + k] : 0.0;
-                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[5 * (size_t)c + k] + f * R[k];
+                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
             }
         } else {
             const double f = rdiv(2.0 / 3.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[5 * (size_t)c + k] : 0.0;
-                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[5 * (size_t)c + k] + f * R[k];
+                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
             }
         }
         cons_to_prim_d(Un, Vnew);
         if (live) {
-            // lane s stores component s, lane 0 also component 4
-            double us = Un[0], vs = Vnew[0];
-            us = (s == 1) ? Un[1] : us;
-            vs = (s == 1) ? Vnew[1] : vs;
-            us = (s == 2) ? Un[2] : us;
-            vs = (s == 2) ? Vnew[2] : vs;
-            us = (s == 3) ? Un[3] : us;
-            vs = (s == 3) ? Vnew[3] : vs;
-            a.Uout[5 * (size_t)c + s] = us;
-            a.Vout[5 * (size_t)c + s] = vs;
-            if (s == 0) {
-                a.Uout[5 * (size_t)c + 4] = Un[4];
-                a.Vout[5 * (size_t)c + 4] = Vnew[4];
-                if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(a.bad_state, 1);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                a.Uout[tix(c, k)] = Un[k];
+                a.Vout[tix(c, k)] = Vnew[k];
             }
+            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(a.bad_state, 1);
         }
     }
     if (a.res_part) {
-        const double r2 = (live && s == 0) ? R[0] * R[0] : 0.0;
+        const double r2 = live ? R[0] * R[0] : 0.0;
         const double v = block_reduce<false>(r2, sh);
         if (threadIdx.x == 0) a.res_part[blockIdx.x] = v;
     }
+}
+
+// prim of a tiled state: V = prim(U)
+__global__ __launch_bounds__(BLOCK) void k_prim(const double *U, double *V, uint32_t N) {
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= N) return;
+    double u[5], v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = U[tix(c, k)];
+    cons_to_prim_d(u, v);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) V[tix(c, k)] = v[k];
 }
 
 }  // namespace nf

@@ -501,6 +501,8 @@ void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order, CellLayo
     L.nbr.assign((size_t)4 * N, 0);
     L.ghost.assign((size_t)4 * N, 0xFFFFFFFFu);
     L.fcode.assign(N, 0);
+    L.bface.clear();
+    L.bbc.clear();
     for (uint32_t d = 0; d < N; ++d) {
         const uint32_t r = L.perm[d];
         const uint32_t *v = &m.cells[4 * (size_t)r];
@@ -522,7 +524,9 @@ void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order, CellLayo
             code |= byte << (8 * s);
             uint32_t w;
             if (j >= N) {
-                w = NBR_BOUNDARY | (uint32_t)(face_bc[f - g.Fint] & 0xFF);
+                w = NBR_BOUNDARY | (uint32_t)L.bface.size();
+                L.bface.push_back(d | (s << 29));
+                L.bbc.push_back((uint8_t)face_bc[f - g.Fint]);
                 L.ghost[4 * (size_t)d + s] = j;
             } else {
                 uint32_t other = (i == r) ? j : i;

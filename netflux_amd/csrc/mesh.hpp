@@ -50,10 +50,13 @@ struct CellLayout {
     std::vector<uint32_t> nbr;    // [N][4] neighbour words (see below)
     std::vector<uint32_t> fcode;  // [N] 4 x 8-bit face codes
     std::vector<uint32_t> ghost;  // [N][4] reference ghost id (boundary slots)
+    std::vector<uint32_t> bface;  // [nb] device cell | slot << 29, per boundary face
+    std::vector<uint8_t> bbc;     // [nb] boundary type of that face
 };
 
 // neighbour word: bit31 = boundary.  interior: bits 29-30 neighbour slot,
-// bits 0-28 neighbour device cell.  boundary: bits 0-7 BC type.
+// bits 0-28 neighbour device cell.  boundary: bits 0-28 boundary face index
+// (position in bface/bbc, numbered in device cell then slot order).
 constexpr uint32_t NBR_BOUNDARY = 0x80000000u;
 constexpr uint32_t NBR_INDEX_MASK = 0x1FFFFFFFu;
 // face code byte: bits 0-1,2-3,4-5 local index of face nodes a,b,c;

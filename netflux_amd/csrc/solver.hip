@@ -93,14 +93,6 @@ __global__ void k_set_flags(int *iscal, int first, int n, double *scal, int set_
     if (t < 2 && clear_scal) scal[t] = 0.0;
 }
 
-__global__ void k_prim(const double *U, double *V, uint32_t N) {
-    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= N) return;
-    double u[5], v[5];
-    for (int k = 0; k < 5; ++k) u[k] = U[5 * (size_t)c + k];
-    cons_to_prim(u, v);
-    for (int k = 0; k < 5; ++k) V[5 * (size_t)c + k] = v[k];
-}
 
 struct DevBuf {
     void *p = nullptr;
@@ -119,6 +111,11 @@ struct nf_ctx {
     std::vector<uint32_t> perm, iperm;  // device <-> reference cell
     FarField<double> fs{};
     int nb = 0;          // blocks of a sweep
+    uint32_t npad = 0;   // N rounded up to whole 64-cell tiles
+    uint32_t nbf = 0;    // boundary faces
+    uint32_t *bface = nullptr;
+    uint8_t *bbc = nullptr;
+    double *Gc = nullptr, *Gf = nullptr;  // ghost states [nbf][5]
     int xcd_remap = 1;
     // device arrays
     double *nodes = nullptr;
@@ -207,10 +204,10 @@ int flush_timing(nf_ctx *ctx) {
 SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux) {
     SweepAArgs a{};
     a.V = ctx->V;
+    a.Gc = ctx->Gc;
     a.FV = ctx->FV;
     a.dt_part = dt ? ctx->dt_part : nullptr;
     a.CFL = ctx->prm.CFL;
-    a.fs = ctx->fs;
     a.stop = ctx->iscal + 2;
     a.xcd_remap = ctx->xcd_remap;
     a.grad_aux = aux ? ctx->grad_aux : nullptr;
@@ -218,7 +215,18 @@ SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux) {
     return a;
 }
 
+int run_ghost(nf_ctx *ctx, bool face) {
+    if (ctx->nbf == 0) return NF_OK;
+    GhostArgs a{ctx->bface, ctx->bbc, ctx->nbf, face ? ctx->FV : ctx->V, face ? ctx->Gf : ctx->Gc, ctx->fs,
+                ctx->iscal + 2};
+    dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
+    if (face) return launch(ctx, "ghost_face", k_ghost<true>, g, b, devmesh(ctx), a);
+    return launch(ctx, "ghost_cell", k_ghost<false>, g, b, devmesh(ctx), a);
+}
+
 int run_sweepA(nf_ctx *ctx, bool dt, bool aux) {
+    int rc;
+    if ((rc = run_ghost(ctx, false))) return rc;
     SweepAArgs a = argsA(ctx, dt, aux);
     const DevMesh m = devmesh(ctx);
     dim3 g(ctx->nb), b(BLOCK);
@@ -231,8 +239,11 @@ int run_sweepA(nf_ctx *ctx, bool dt, bool aux) {
 
 int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double *Uout, bool res,
                bool aux) {
+    int rc;
+    if ((rc = run_ghost(ctx, true))) return rc;
     SweepBArgs a{};
     a.FV = ctx->FV;
+    a.Gf = ctx->Gf;
     a.U = U;
     a.Uo = Uo;
     a.Uout = Uout;
@@ -243,7 +254,6 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double 
     a.bad_flux = ctx->iscal + 4;
     a.check = ctx->prm.check_physical_validity;
     a.bad_state = ctx->iscal + 5;
-    a.fs = ctx->fs;
     a.stop = ctx->iscal + 2;
     a.xcd_remap = ctx->xcd_remap;
     a.R_aux = aux ? ctx->R_aux : nullptr;
@@ -347,15 +357,29 @@ int ensure_hist(nf_ctx *ctx, int need) {
     return NF_OK;
 }
 
-// reference-order AoS [N][rows] <-> device-order AoS [N][rows]
+// reference-order AoS [N][5] <-> device-order tiles [tile][5][64]
 int upload_aos(nf_ctx *ctx, const double *aos, double *dev) {
     const uint32_t N = ctx->N;
-    std::vector<double> d((size_t)5 * N);
-    for (uint32_t c = 0; c < N; ++c) std::memcpy(&d[5 * (size_t)c], aos + 5 * (size_t)ctx->perm[c], 40);
+    std::vector<double> d((size_t)5 * ctx->npad, 0.0);
+    for (uint32_t c = 0; c < N; ++c)
+        for (int k = 0; k < 5; ++k)
+            d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)] = aos[5 * (size_t)ctx->perm[c] + k];
     HIPC(hipMemcpy(dev, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
     return NF_OK;
 }
 
+int download_tiled(nf_ctx *ctx, const double *dev, double *aos) {
+    const uint32_t N = ctx->N;
+    std::vector<double> d((size_t)5 * ctx->npad);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < N; ++c)
+        for (int k = 0; k < 5; ++k)
+            aos[5 * (size_t)ctx->perm[c] + k] = d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)];
+    return NF_OK;
+}
+
+// device-order AoS [N][rows] -> reference-order AoS
 int download_aos(nf_ctx *ctx, const double *dev, int rows, double *aos) {
     const uint32_t N = ctx->N;
     std::vector<double> d((size_t)rows * N);
@@ -419,7 +443,8 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     ctx->perm = L.perm;
     ctx->iperm = L.iperm;
     ctx->fs = make_farfield<double>(prm->primvars_inf);
-    ctx->nb = (int)((g.N + CELLS - 1) / CELLS);
+    ctx->nb = (int)((g.N + BLOCK - 1) / BLOCK);
+    ctx->npad = (g.N + TILE - 1) / TILE * TILE;
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const uint32_t N = g.N;
     int rc;
@@ -427,10 +452,20 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     if ((rc = dalloc(ctx, &ctx->cnodes, 4 * (size_t)N))) return rc;
     if ((rc = dalloc(ctx, &ctx->nbr, 4 * (size_t)N))) return rc;
     if ((rc = dalloc(ctx, &ctx->fcode, (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->Ua, 5 * (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->Ub, 5 * (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->V, 5 * (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->FV, 20 * (size_t)N))) return rc;
+    const size_t NP = ctx->npad;
+    if ((rc = dalloc(ctx, &ctx->Ua, 5 * NP))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Ub, 5 * NP))) return rc;
+    if ((rc = dalloc(ctx, &ctx->V, 5 * NP))) return rc;
+    if ((rc = dalloc(ctx, &ctx->FV, 20 * NP))) return rc;
+    ctx->nbf = (uint32_t)L.bface.size();
+    if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+    if (ctx->nbf) {
+        HIPC(hipMemcpy(ctx->bface, L.bface.data(), L.bface.size() * 4, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(ctx->bbc, L.bbc.data(), L.bbc.size(), hipMemcpyHostToDevice));
+    }
     if ((rc = dalloc(ctx, &ctx->dt_part, (size_t)ctx->nb))) return rc;
     if ((rc = dalloc(ctx, &ctx->res_part, (size_t)ctx->nb))) return rc;
     ctx->nb2 = (ctx->nb + RCHUNK - 1) / RCHUNK;
@@ -653,7 +688,7 @@ int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
 
 int nf_get_state(nf_ctx *ctx, double *U) {
     if (!ctx || !U) return NF_E_ARG;
-    return download_aos(ctx, ctx->Ua, 5, U);
+    return download_tiled(ctx, ctx->Ua, U);
 }
 
 // Evaluates the flux balance of the current state once (Solver::
@@ -682,13 +717,13 @@ int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
     if (!ctx || !out) return NF_E_ARG;
     switch (which) {
     case NF_FIELD_PRIM:
-        return download_aos(ctx, ctx->V, 5, out);
+        return download_tiled(ctx, ctx->V, out);
     case NF_FIELD_STATE_OLD: {
         if (ctx->uold_zero) {
             std::memset(out, 0, sizeof(double) * 5 * ctx->N);
             return NF_OK;
         }
-        return download_aos(ctx, ctx->Ua, 5, out);
+        return download_tiled(ctx, ctx->Ua, out);
     }
     case NF_FIELD_LIMITER:
     case NF_FIELD_GRADIENT:
