@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnetflux.so")
+# NETFLUX_LIB overrides the library path (e.g. to compare builds)
+LIB_PATH = os.environ.get("NETFLUX_LIB") or os.path.join(_HERE, "libnetflux.so")
 
 NF_OK, NF_E_ARG, NF_E_MESH, NF_E_HIP, NF_E_INVALID_DT, NF_E_NONPHYSICAL, NF_E_NOGPU = range(7)
 NF_EXPLICIT_EULER, NF_TVD_RK3 = 0, 1

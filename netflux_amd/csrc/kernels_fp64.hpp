@@ -34,6 +34,9 @@
 namespace nf {
 
 constexpr int BLOCK = 256;  // threads = cells per block
+#ifndef NF_SWEEP_WAVES
+#define NF_SWEEP_WAVES 1   // minimum waves per SIMD requested for the sweeps
+#endif
 constexpr int TILE = 64;    // cells per layout tile
 
 __device__ __forceinline__ size_t tix(uint32_t c, int k) {  // [tile][5][64]
@@ -82,6 +85,14 @@ __device__ __forceinline__ double dmin(double a, double b) {
     double r;
     asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
+}
+
+// x with its sign bit xor-ed with m (m = 0 or 0x80000000 on the high word):
+// exact negation without a select.  x - t == x + (-t) and s*(-x) == -(s*x)
+// hold exactly in IEEE arithmetic, which lets the owner/neighbour side of a
+// face be handled by sign flips.
+__device__ __forceinline__ double flip(double x, uint32_t m) {
+    return __hiloint2double(__double2hiint(x) ^ (int)m, __double2loint(x));
 }
 
 // Division by a divisor used several times.  The f64 divide of the target is
@@ -142,6 +153,35 @@ __device__ __forceinline__ void rusanov_d(const double *UL, const double *UR, V3
     const double h = area * 0.5;
 #pragma unroll
     for (int k = 0; k < 5; ++k) F[k] = h * ((FR[k] + FL[k]) - sm * (UR[k] - UL[k]));
+}
+
+// The same flux seen from one cell: UO is the cell's own face state, UN the
+// neighbour's, mneg = 0 if the cell is the reference owner i (UL = UO) and
+// the sign bit otherwise (UL = UN).  F(UL,UR) = h((FR+FL) - s(UR-UL)); with
+// UR-UL = -(UO-UN) for the owner this is h((FR+FL) + s(UO-UN)), and
+// h((FR+FL) - s(UO-UN)) otherwise -- bit-identical to rusanov_d(UL, UR).
+__device__ __forceinline__ void rusanov_side(const double *UO, const double *UN, V3d S, uint32_t mneg, double *F) {
+    const double z = (S.x * S.x + S.y * S.y) + S.z * S.z;
+    const double area = sqrt(z);
+    const Rcp ia = rcp_of(area);
+    V3d n = S;
+    if (z > 0.0) n = {rdiv(S.x, ia), rdiv(S.y, ia), rdiv(S.z, ia)};
+    const Rcp iO = rcp_of(UO[0]), iN = rcp_of(UN[0]);
+    const double G = Consts<double>::GAMMA, GM1 = Consts<double>::GM1;
+    const double pO = GM1 * (UO[4] - rdiv(0.5, iO) * ((UO[1] * UO[1] + UO[2] * UO[2]) + UO[3] * UO[3]));
+    const double pN = GM1 * (UN[4] - rdiv(0.5, iN) * ((UN[1] * UN[1] + UN[2] * UN[2]) + UN[3] * UN[3]));
+    const double vnO = rdiv((UO[1] * n.x + UO[2] * n.y) + UO[3] * n.z, iO);
+    const double vnN = rdiv((UN[1] * n.x + UN[2] * n.y) + UN[3] * n.z, iN);
+    const double cO = sqrt(rdiv(G * pO, iO));
+    const double cN = sqrt(rdiv(G * pN, iN));
+    const double sm = dmax(fabs(vnO) + cO, fabs(vnN) + cN);
+    const double FO[5] = {vnO * UO[0], vnO * UO[1] + pO * n.x, vnO * UO[2] + pO * n.y,
+                          vnO * UO[3] + pO * n.z, vnO * (UO[4] + pO)};
+    const double FN[5] = {vnN * UN[0], vnN * UN[1] + pN * n.x, vnN * UN[2] + pN * n.y,
+                          vnN * UN[3] + pN * n.z, vnN * (UN[4] + pN)};
+    const double h = area * 0.5;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) F[k] = h * ((FO[k] + FN[k]) + flip(sm * (UO[k] - UN[k]), mneg));
 }
 
 // BoundaryCondition::calc_ghost_val (src/Numerics.cpp:59-171); slip wall with
@@ -227,6 +267,40 @@ __device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, uin
     }
 }
 
+#ifndef NF_LDS_NODES
+#define NF_LDS_NODES 1
+#endif
+// The four nodes of each lane's cell staged in LDS, [12][BLOCK] (node q,
+// coordinate d at row 3q+d); every lane reads only its own column, so no
+// barrier is needed, and a wave's reads of one row hit 64 distinct banks.
+__device__ __forceinline__ void stage_nodes(double *lds, const CellGeo &g) {
+    const uint32_t t = threadIdx.x;
+    const V3d P[4] = {g.p0, g.p1, g.p2, g.p3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        lds[(3 * q + 0) * BLOCK + t] = P[q].x;
+        lds[(3 * q + 1) * BLOCK + t] = P[q].y;
+        lds[(3 * q + 2) * BLOCK + t] = P[q].z;
+    }
+}
+__device__ __forceinline__ V3d lds_node(const double *lds, uint32_t q) {
+    const uint32_t t = threadIdx.x;
+    return {lds[(3 * q + 0) * BLOCK + t], lds[(3 * q + 1) * BLOCK + t], lds[(3 * q + 2) * BLOCK + t]};
+}
+// face_geo with the face nodes read from the staged LDS copy
+__device__ __forceinline__ void face_geo_lds(const DevMesh &m, const CellGeo &g, const double *lds, uint32_t byte,
+                                             V3d &S, V3d *r) {
+    const V3d a = lds_node(lds, byte & 3u), b = lds_node(lds, (byte >> 2) & 3u), c = lds_node(lds, (byte >> 4) & 3u);
+    S = vscl(0.5, vcross(vsub(b, a), vsub(c, a)));
+    if (byte & 0x40u) S = vscl(-1, S);
+    if (r) {
+        const V3d s = vadd(vadd(a, b), c);
+        const Rcp i3 = rcp_of(m.three);
+        const V3d fc = {rdiv(s.x, i3), rdiv(s.y, i3), rdiv(s.z, i3)};
+        *r = vsub(fc, g.cen);
+    }
+}
+
 // wave + block min / sum over all threads; result valid in thread 0
 template <bool MIN>
 __device__ __forceinline__ double block_reduce(double v, double *sh) {
@@ -293,7 +367,7 @@ struct SweepAArgs {
 
 // ORDER: 0 first order, 1 second order without limiter, 2 second order Barth
 template <int ORDER>
-__global__ __launch_bounds__(BLOCK) void k_sweepA(DevMesh m, SweepAArgs a) {
+__global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[BLOCK / 64];
     if (*a.stop) return;
     const uint32_t N = m.N;
@@ -301,17 +375,36 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA(DevMesh m, SweepAArgs a) {
     const bool live = c0 < N;
     const uint32_t c = live ? c0 : N - 1;  // dead lanes mirror the last cell, store nothing
 
-    CellGeo g;
-    load_cell(m, c, g);
+    // all loads first: own state, neighbour states (or ghost states), nodes
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    V3d S[4], r[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo(m, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     double Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    double Vn[4][5];
+    if (ORDER != 0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t idx = w[s] & 0x1FFFFFFFu;
+            const double *src = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
+            const size_t stride = (w[s] & 0x80000000u) ? 1 : TILE;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
+        }
+    }
+    CellGeo g;
+    load_cell(m, c, g);
+    V3d S[4], r[4];
+#if NF_LDS_NODES
+    __shared__ double lds_nodes[12 * BLOCK];
+    stage_nodes(lds_nodes, g);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) face_geo_lds(m, g, lds_nodes, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+#else
+#pragma unroll
+    for (int s = 0; s < 4; ++s) face_geo(m, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+#endif
     __builtin_amdgcn_sched_barrier(0);
 
     if (ORDER == 0) {
@@ -326,30 +419,22 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA(DevMesh m, SweepAArgs a) {
         // Barth factor and the face values depend on variable k only.
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            double Vn[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const uint32_t idx = w[s] & 0x1FFFFFFFu;
-                Vn[s] = (w[s] & 0x80000000u) ? a.Gc[5 * (size_t)idx + k] : a.V[tix(idx, k)];
-            }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
             double g0 = 0.0, g1 = 0.0, g2 = 0.0;
             double vmax = -DBL_MAX, vmin = DBL_MAX;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const bool owner = (code >> (8 * s + 7)) & 1u;
-                const double vi = owner ? Vc[k] : Vn[s], vj = owner ? Vn[s] : Vc[k];
-                const double uf = 0.5 * (vi + vj);
-                const double t0 = rdiv(uf * S[s].x, g.ivol), t1 = rdiv(uf * S[s].y, g.ivol),
-                             t2 = rdiv(uf * S[s].z, g.ivol);
-                g0 = owner ? g0 + t0 : g0 - t0;
-                g1 = owner ? g1 + t1 : g1 - t1;
-                g2 = owner ? g2 + t2 : g2 - t2;
+                // 0.5 (V_i + V_j) is symmetric in i, j; the non-owner subtracts
+                const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
+                const double uf = 0.5 * (Vc[k] + Vn[s][k]);
+                g0 = g0 + flip(rdiv(uf * S[s].x, g.ivol), mneg);
+                g1 = g1 + flip(rdiv(uf * S[s].y, g.ivol), mneg);
+                g2 = g2 + flip(rdiv(uf * S[s].z, g.ivol), mneg);
                 if (ORDER == 2) {
                     // neighbour min/max (include/Numerics.hpp:276-291)
-                    vmax = dmax(vmax, dmax(vi, vj));
-                    vmin = dmin(vmin, dmin(vi, vj));
+                    vmax = dmax(vmax, dmax(Vc[k], Vn[s][k]));
+                    vmin = dmin(vmin, dmin(Vc[k], Vn[s][k]));
                 }
             }
             double delta[4];
@@ -416,7 +501,7 @@ struct SweepBArgs {
     double *R_aux;     // [N][5] or null: store R, skip the update
 };
 
-__global__ __launch_bounds__(BLOCK) void k_sweepB(DevMesh m, SweepBArgs a) {
+__global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, SweepBArgs a) {
     __shared__ double sh[BLOCK / 64];
     if (*a.stop) return;
     const uint32_t N = m.N;
@@ -424,19 +509,53 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB(DevMesh m, SweepBArgs a) {
     const bool live = c0 < N;
     const uint32_t c = live ? c0 : N - 1;
 
-    CellGeo g;
-    load_cell(m, c, g);
+#ifndef NF_B_PRELOAD
+#define NF_B_PRELOAD 1
+#endif
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    // face values of both sides of the four faces, all loads issued up front
+    double Vo4[4][5], Vn4[4][5];
+#if NF_B_PRELOAD
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
+        const bool bnd = w[s] & 0x80000000u;
+        const double *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
+        const size_t stride = bnd ? 1 : TILE;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Vo4[s][k] = a.FV[fix(c, s, k)];
+            Vn4[s][k] = src[k * stride];
+        }
+    }
+#endif
+    CellGeo g;
+    load_cell(m, c, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#if NF_LDS_NODES
+    __shared__ double lds_nodes[12 * BLOCK];
+    stage_nodes(lds_nodes, g);
+#endif
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t byte = (code >> (8 * s)) & 0xFFu;
         const bool owner = byte & 0x80u;
         V3d S;
+#if NF_LDS_NODES
+        face_geo_lds(m, g, lds_nodes, byte, S, nullptr);
+#else
         face_geo(m, g, byte, S, nullptr);
+#endif
         double Vo[5], Vn[5];
+#if NF_B_PRELOAD
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Vo[k] = Vo4[s][k];
+            Vn[k] = Vn4[s][k];
+        }
+#else
         const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
         const bool bnd = w[s] & 0x80000000u;
 #pragma unroll
@@ -444,18 +563,15 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB(DevMesh m, SweepBArgs a) {
             Vo[k] = a.FV[fix(c, s, k)];
             Vn[k] = bnd ? a.Gf[5 * (size_t)idx + k] : a.FV[fix(idx, sn, k)];
         }
+#endif
         // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
-        double UL[5], UR[5], F[5];
-        if (owner) {
-            prim_to_cons(Vo, UL);
-            prim_to_cons(Vn, UR);
-        } else {
-            prim_to_cons(Vn, UL);
-            prim_to_cons(Vo, UR);
-        }
-        rusanov_d(UL, UR, S, F);
+        double UO[5], UN[5], F[5];
+        prim_to_cons(Vo, UO);
+        prim_to_cons(Vn, UN);
+        const uint32_t mneg = owner ? 0u : 0x80000000u;
+        rusanov_side(UO, UN, S, mneg, F);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) R[k] = owner ? R[k] - F[k] : R[k] + F[k];  // R_i -= F, R_j += F
+        for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
         __builtin_amdgcn_sched_barrier(0);
     }
     if (live && a.check) {

@@ -662,3 +662,20 @@ double or_time(void *h) { return static_cast<Solver *>(h)->time; }
 const char *or_error(void *h) { return static_cast<Solver *>(h)->err.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Unit-level entry points of the oracle's physics (tests only)
+extern "C" {
+
+void or_prim_to_cons(const double *V, double *U) { prim_to_cons(V, U); }
+void or_cons_to_prim(const double *U, double *V) { cons_to_prim(U, V); }
+void or_rusanov(const double *UL, const double *UR, const double *S, double *F) {
+    rusanov(UL, UR, Vec3{{S[0], S[1], S[2]}}, F);
+}
+// type: 0 no-slip, 1 slip, 2 farfield with freestream V_fs
+void or_ghost(int type, const double *V_fs, const double *V_domain, const double *S, double *V_ghost) {
+    BCFarField ff(V_fs);
+    bc_ghost(type, ff, V_domain, V_ghost, Vec3{{S[0], S[1], S[2]}});
+}
+
+}  // extern "C"

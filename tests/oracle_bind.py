@@ -137,3 +137,32 @@ class Oracle:
 
     def time(self):
         return lib().or_time(self._h)
+
+
+def _vec(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def prim_to_cons(V):
+    V, U = _vec(V), np.zeros(5)
+    lib().or_prim_to_cons(_p(V, C.c_double), _p(U, C.c_double))
+    return U
+
+
+def cons_to_prim(U):
+    U, V = _vec(U), np.zeros(5)
+    lib().or_cons_to_prim(_p(U, C.c_double), _p(V, C.c_double))
+    return V
+
+
+def rusanov(UL, UR, S):
+    UL, UR, S, F = _vec(UL), _vec(UR), _vec(S), np.zeros(5)
+    lib().or_rusanov(_p(UL, C.c_double), _p(UR, C.c_double), _p(S, C.c_double), _p(F, C.c_double))
+    return F
+
+
+def ghost(bc_type, V_fs, V_domain, S):
+    V_fs, Vd, S, Vg = _vec(V_fs), _vec(V_domain), _vec(S), np.zeros(5)
+    lib().or_ghost(int(bc_type), _p(V_fs, C.c_double), _p(Vd, C.c_double), _p(S, C.c_double),
+                   _p(Vg, C.c_double))
+    return Vg
