@@ -1,0 +1,177 @@
+// C++ host classes with the reference's solver surface, over the C ABI of
+// include/netflux.h.  Header-only; link with libnetflux.so.
+//
+//   netflux::Config        the option and run-state subset of Config
+//                          (include/Config.hpp:8-175) the step depends on
+//   netflux::EulerSolver   drop-in for EulerSolver (include/Solver.hpp:52-79):
+//                          step(const Config&), calc_timestep(Config&),
+//                          get_solver_data(), get_solver_type()
+//   netflux::SolverData    get_solution(), get_primvars(),
+//                          get_primvars_gradient(), get_primvars_limiter(),
+//                          get_flux_balance() as AoS VecField copies
+//   netflux::Driver        the time loop of src/Driver.cpp:31-57 (no VTK)
+//
+// Errors are thrown as std::runtime_error carrying the reference's messages
+// (src/Solver.cpp:267-268, src/SolverData.cpp:73-78).
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/netflux.h"
+
+namespace netflux {
+
+enum class TimeScheme { ExplicitEuler, TVD_RK3 };
+enum class SpatialOrder { First, Second };
+enum class Limiter { NONE, Barth };
+enum class InviscidFluxScheme { Rusanov, HLLC };
+enum class BoundaryType { NoSlipWall, SlipWall, FarField };
+
+struct Config {
+    TimeScheme time_scheme = TimeScheme::TVD_RK3;
+    SpatialOrder spatial_order = SpatialOrder::Second;
+    Limiter limiter = Limiter::Barth;
+    InviscidFluxScheme inv_flux_scheme = InviscidFluxScheme::Rusanov;
+    double CFL = 0.5;
+    double primvars_inf[5] = {1.225, 0.0, 0.0, 0.0, 101325.0};  // standard air defaults
+    bool check_physical_validity = false;
+    size_t n_timesteps = 1;
+    bool rk3_zero_old_first_step = true;
+    int precision = NF_PREC_FP64;
+    int cell_order = NF_ORDER_MORTON;
+    // run state (Config::delta_time / time / timestep)
+    double delta_time = 0.0;
+    double time = 0.0;
+    size_t timestep = 0;
+
+    double get_delta_time() const { return delta_time; }
+    void set_delta_time(double dt) { delta_time = dt; }
+    double get_time() const { return time; }
+    void set_time(double t) { time = t; }
+    size_t get_timestep() const { return timestep; }
+    void set_timestep(size_t n) { timestep = n; }
+    size_t get_n_timesteps() const { return n_timesteps; }
+
+    nf_params to_params() const {
+        nf_params p{};
+        p.time_scheme = time_scheme == TimeScheme::TVD_RK3 ? NF_TVD_RK3 : NF_EXPLICIT_EULER;
+        p.spatial_order = spatial_order == SpatialOrder::Second ? NF_SECOND_ORDER : NF_FIRST_ORDER;
+        p.limiter = limiter == Limiter::Barth ? NF_LIMITER_BARTH : NF_LIMITER_NONE;
+        p.inv_flux_scheme = inv_flux_scheme == InviscidFluxScheme::Rusanov ? NF_FLUX_RUSANOV : NF_FLUX_HLLC;
+        p.CFL = CFL;
+        for (int k = 0; k < 5; ++k) p.primvars_inf[k] = primvars_inf[k];
+        p.check_physical_validity = check_physical_validity;
+        p.rk3_zero_old_first_step = rk3_zero_old_first_step;
+        p.precision = precision;
+        p.cell_order = cell_order;
+        return p;
+    }
+};
+
+// AoS [cells][rows] doubles, the layout of VecField / GradField
+struct VecField {
+    uint32_t n = 0, rows = 0;
+    std::vector<double> data;
+    double operator()(uint32_t i, uint32_t k) const { return data[(size_t)i * rows + k]; }
+    uint32_t size() const { return n; }
+};
+
+inline void check(int rc, const nf_ctx *ctx, const char *what) {
+    if (rc == NF_OK) return;
+    const std::string msg = ctx ? nf_last_error(ctx) : "";
+    if (rc == NF_E_INVALID_DT || rc == NF_E_NONPHYSICAL) throw std::runtime_error(msg);
+    throw std::runtime_error(std::string(what) + " failed (status " + std::to_string(rc) + "): " + msg);
+}
+
+class EulerSolver;
+
+class SolverData {
+    const EulerSolver *s_;
+
+  public:
+    explicit SolverData(const EulerSolver *s) : s_(s) {}
+    VecField get_solution() const;
+    VecField get_primvars() const;
+    VecField get_primvars_gradient() const;  // column-major 5x3 per cell
+    VecField get_primvars_limiter() const;
+    VecField get_flux_balance() const;
+};
+
+class EulerSolver {
+    nf_ctx *ctx_ = nullptr;
+    nf_fv_sizes sizes_{};
+    SolverData data_{this};
+
+  public:
+    EulerSolver(const Config &config, const nf_mesh_desc &mesh) {
+        const nf_params p = config.to_params();
+        check(nf_create(&mesh, &p, &ctx_), nullptr, "nf_create");
+        check(nf_ctx_sizes(ctx_, &sizes_), ctx_, "nf_ctx_sizes");
+    }
+    ~EulerSolver() { nf_destroy(ctx_); }
+    EulerSolver(const EulerSolver &) = delete;
+    EulerSolver &operator=(const EulerSolver &) = delete;
+
+    void calc_timestep(Config &config) {
+        double dt = 0.0;
+        check(nf_calc_timestep(ctx_, &dt), ctx_, "nf_calc_timestep");
+        config.set_delta_time(dt);
+    }
+    void step(const Config &config) { check(nf_step(ctx_, config.get_delta_time()), ctx_, "nf_step"); }
+    const SolverData &get_solver_data() const { return data_; }
+    const char *get_solver_type() const { return "Euler"; }
+
+    // additions: state injection and the device-resident loop
+    void set_state(const std::vector<double> &U, const std::vector<double> *U_old = nullptr) {
+        check(nf_set_state(ctx_, U.data(), U_old ? U_old->data() : nullptr), ctx_, "nf_set_state");
+    }
+    nf_run_stats run(int n_steps, double residual_drop = 0.0, int check_every = 10) {
+        nf_run_opts o{n_steps, residual_drop, check_every, 1};
+        nf_run_stats st{};
+        check(nf_run(ctx_, &o, &st), ctx_, "nf_run");
+        return st;
+    }
+    void eval_flux_balance() { check(nf_eval_flux_balance(ctx_), ctx_, "nf_eval_flux_balance"); }
+    uint32_t n_cells() const { return sizes_.n_interior_cells; }
+    nf_ctx *handle() const { return ctx_; }
+};
+
+inline VecField fetch(const EulerSolver *s, int which, uint32_t rows) {
+    VecField f;
+    f.n = s->n_cells();
+    f.rows = rows;
+    f.data.resize((size_t)f.n * rows);
+    if (which < 0) check(nf_get_state(s->handle(), f.data.data()), s->handle(), "nf_get_state");
+    else check(nf_get_field(s->handle(), which, f.data.data()), s->handle(), "nf_get_field");
+    return f;
+}
+inline VecField SolverData::get_solution() const { return fetch(s_, -1, 5); }
+inline VecField SolverData::get_primvars() const { return fetch(s_, NF_FIELD_PRIM, 5); }
+inline VecField SolverData::get_primvars_gradient() const { return fetch(s_, NF_FIELD_GRADIENT, 15); }
+inline VecField SolverData::get_primvars_limiter() const { return fetch(s_, NF_FIELD_LIMITER, 5); }
+inline VecField SolverData::get_flux_balance() const { return fetch(s_, NF_FIELD_RESIDUAL, 5); }
+
+// src/Driver.cpp:31-57 without output
+class Driver {
+    Config &config_;
+    EulerSolver solver_;
+
+  public:
+    Driver(Config &config, const nf_mesh_desc &mesh) : config_(config), solver_(config, mesh) {}
+    EulerSolver &solver() { return solver_; }
+    void solve() {
+        config_.set_timestep(0);
+        config_.set_time(0.0);
+        while (true) {
+            solver_.calc_timestep(config_);
+            solver_.step(config_);
+            config_.set_timestep(config_.get_timestep() + 1);
+            config_.set_time(config_.get_time() + config_.get_delta_time());
+            if (config_.get_timestep() >= config_.get_n_timesteps()) break;
+        }
+    }
+};
+
+}  // namespace netflux

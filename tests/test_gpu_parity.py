@@ -109,3 +109,51 @@ def test_graph_equals_eager():
     assert sa["steps_done"] == sb["steps_done"] == 30
     np.testing.assert_array_equal(a.get_solver_data().get_solution(), b.get_solver_data().get_solution())
     np.testing.assert_array_equal(a.residuals(), b.residuals())
+
+
+def test_cpp_driver_matches_oracle(tmp_path):
+    """The C++ host classes (netflux_solver.hpp) in the Driver loop order."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(nf.LIB_PATH), "netflux_driver")
+    out = tmp_path / "U.bin"
+    print("[case] netflux_driver 16 100 sod", flush=True)
+    r = subprocess.run([exe, "16", "100", "sod", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    mesh = nf.Mesh.box(16)
+    Ug = np.fromfile(out, dtype=np.float64).reshape(-1, 5)
+    ora = Oracle.from_mesh(mesh, sod_config())
+    ora.set_state(sod_state(mesh))
+    ora.run(100)
+    np.testing.assert_array_equal(Ug, ora.get("U"))
+
+
+def test_mixed_patches_noslip_farfield():
+    """All three boundary types in one mesh, explicit and RK3 steps."""
+    mesh = nf.Mesh.box((6, 5, 4), jitter=0.12, seed=31,
+                       patches=("FarField", "NoSlipWall", "SlipWall", "NoSlipWall", "FarField", "SlipWall"))
+    cfg = pulse_config(mach=0.3)
+    gpu, ora = _pair(mesh, cfg, pulse_state(mesh, mach=0.3), uold_equals_u=True)
+    cfg.rk3_zero_old_first_step = False
+    for _ in range(15):
+        gpu.calc_timestep(cfg)
+        gpu.step(cfg)
+    ora.run(15)
+    assert rel_l2(gpu.get_solver_data().get_solution(), ora.get("U")) <= TOL_REL_L2
+
+
+def test_residual_history_matches_oracle():
+    """r_k = ||R_rho||_2 at the first stage of step k."""
+    mesh = nf.Mesh.box(6, jitter=0.1, seed=4)
+    cfg = sod_config()
+    U0 = sod_state(mesh)
+    gpu, ora = _pair(mesh, cfg, U0)
+    gpu.run(5, use_graph=False)
+    res = gpu.residuals()
+    assert len(res) == 5
+    expect = []
+    for _ in range(5):
+        ora.eval_flux_balance()
+        expect.append(np.linalg.norm(ora.get("R")[:, 0]))
+        ora.step(ora.calc_timestep())
+    np.testing.assert_allclose(res, expect, rtol=1e-13)
