@@ -319,6 +319,7 @@ __device__ __forceinline__ double block_reduce(double v, double *sh) {
     return v;
 }
 
+#ifndef NF_DEVICE_HELPERS_ONLY
 // ---------------------------------------------------------------------------
 // boundary ghost states.  bface[b] = device cell | slot << 29, bbc[b] = type.
 struct GhostArgs {
@@ -636,5 +637,7 @@ __global__ __launch_bounds__(BLOCK) void k_prim(const double *U, double *V, uint
 #pragma unroll
     for (int k = 0; k < 5; ++k) V[tix(c, k)] = v[k];
 }
+
+#endif  // NF_DEVICE_HELPERS_ONLY
 
 }  // namespace nf

@@ -1,0 +1,359 @@
+// Mixed-precision stage kernels: FP32 gradient, limiter, reconstruction and
+// Rusanov flux on FP32 copies of the primitive and face states; FP64 state,
+// flux-balance accumulation, RK combination, time step and residual norms.
+//
+// Same mapping, layouts and face order as the FP64 kernels
+// (kernels_fp64.hpp), with FP32 V [tile][5][64], FV [tile][4][5][64] and
+// ghost states.  Face geometry is formed in FP64 from the nodes and rounded
+// once: S from the face's own node order, so the two cells of a face hold
+// bit-identical FP32 normals, and each face flux is evaluated by both cells
+// with identical FP32 arithmetic -- the FP64 flux balance conserves exactly.
+// This header is compiled with floating-point contraction enabled.
+#pragma once
+#include "kernels_fp64.hpp"
+
+namespace nf {
+namespace mx {
+
+// kernel arguments
+struct GhostArgsMx {
+    const uint32_t *bface;
+    const uint8_t *bbc;
+    uint32_t nb;
+    const float *src;  // V32 [tile][5][64] (cell) or FV32 (face)
+    float *out;        // [nb][5]
+    FarField<float> fs;
+    const int *stop;
+};
+
+struct SweepAArgsMx {
+    const float *V;
+    const float *Gc;
+    float *FV;
+    double *dt_part;
+    double CFL;
+    const int *stop;
+    int xcd_remap;
+    double *grad_aux, *psi_aux;
+};
+
+struct SweepBArgsMx {
+    const float *FV;
+    const float *Gf;
+    const double *U;
+    const double *Uo;
+    double *Uout;
+    float *Vout;
+    const double *dt;
+    int kind;
+    double *res_part;
+    int *bad_flux;
+    int check;
+    int *bad_state;
+    const int *stop;
+    int xcd_remap;
+    double *R_aux;
+};
+
+#ifdef NF_MIXED_KERNELS
+using V3f = Vec3T<float>;
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float fflip(float x, uint32_t m) { return __int_as_float(__float_as_int(x) ^ (int)m); }
+
+__device__ __forceinline__ V3f to_f(V3d v) { return {(float)v.x, (float)v.y, (float)v.z}; }
+
+// S (FP32, from FP64 edge vectors of the face's own node order) and, if r is
+// given, the FP32 centroid-to-face vector
+__device__ __forceinline__ void face_geo_mx(const double *lds, const CellGeo &g, uint32_t byte, V3f &S, V3f *r) {
+    const V3d a = lds_node(lds, byte & 3u), b = lds_node(lds, (byte >> 2) & 3u), c = lds_node(lds, (byte >> 4) & 3u);
+    const V3f e1 = to_f(vsub(b, a)), e2 = to_f(vsub(c, a));
+    S = {0.5f * (e1.y * e2.z - e1.z * e2.y), 0.5f * (e1.z * e2.x - e1.x * e2.z), 0.5f * (e1.x * e2.y - e1.y * e2.x)};
+    if (byte & 0x40u) S = {-S.x, -S.y, -S.z};
+    if (r) {
+        const V3d s = vadd(vadd(a, b), c);
+        *r = to_f(vsub({s.x / 3, s.y / 3, s.z / 3}, g.cen));
+    }
+}
+
+__device__ __forceinline__ void prim_to_cons_f(const float *V, float *U) {
+    U[0] = V[0];
+    U[1] = V[0] * V[1];
+    U[2] = V[0] * V[2];
+    U[3] = V[0] * V[3];
+    U[4] = Consts<float>::GM1_INV * V[4] + 0.5f * V[0] * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
+}
+
+// Rusanov flux seen from one cell (see rusanov_side): F for UL = own if the
+// cell is the owner (mneg = 0), UL = neighbour otherwise
+__device__ __forceinline__ void rusanov_side_f(const float *UO, const float *UN, V3f S, uint32_t mneg, float *F) {
+    const float z = S.x * S.x + S.y * S.y + S.z * S.z;
+    const float area = fsqrt(z);
+    const float ia = frcp(area);
+    const V3f n = {S.x * ia, S.y * ia, S.z * ia};
+    const float iO = frcp(UO[0]), iN = frcp(UN[0]);
+    const float G = Consts<float>::GAMMA, GM1 = Consts<float>::GM1;
+    const float pO = GM1 * (UO[4] - 0.5f * iO * (UO[1] * UO[1] + UO[2] * UO[2] + UO[3] * UO[3]));
+    const float pN = GM1 * (UN[4] - 0.5f * iN * (UN[1] * UN[1] + UN[2] * UN[2] + UN[3] * UN[3]));
+    const float vnO = (UO[1] * n.x + UO[2] * n.y + UO[3] * n.z) * iO;
+    const float vnN = (UN[1] * n.x + UN[2] * n.y + UN[3] * n.z) * iN;
+    const float sm = fmaxf(fabsf(vnO) + fsqrt(G * pO * iO), fabsf(vnN) + fsqrt(G * pN * iN));
+    const float FO[5] = {vnO * UO[0], vnO * UO[1] + pO * n.x, vnO * UO[2] + pO * n.y, vnO * UO[3] + pO * n.z,
+                         vnO * (UO[4] + pO)};
+    const float FN[5] = {vnN * UN[0], vnN * UN[1] + pN * n.x, vnN * UN[2] + pN * n.y, vnN * UN[3] + pN * n.z,
+                         vnN * (UN[4] + pN)};
+    const float h = 0.5f * area;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float d = sm * (UO[k] - UN[k]);
+        const float c = FO[k] + FN[k];
+        F[k] = h * (c + fflip(d, mneg));
+    }
+}
+
+__device__ __forceinline__ float barth_phi_f(float delta, float vmax, float vmin, float v) {
+    const float d2 = float(isign(delta)) * (fabsf(delta) + FLT_EPSILON);
+    const float num = (d2 > 0.0f) ? vmax - v : vmin - v;
+    return (d2 != 0.0f) ? fminf(1.0f, num / d2) : 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+
+template <bool FACE>
+__global__ __launch_bounds__(BLOCK) void k_ghost_mx(DevMesh m, GhostArgsMx a) {
+    __shared__ double lds[12 * BLOCK];
+    if (*a.stop) return;
+    const uint32_t b0 = blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = b0 < a.nb;
+    const uint32_t b = live ? b0 : a.nb - 1;
+    const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
+    CellGeo g;
+    load_cell(m, c, g);
+    stage_nodes(lds, g);
+    V3f S;
+    face_geo_mx(lds, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    float Vd[5], Vg[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
+    ghost_value<float>((int)a.bbc[b], Vd, S, a.fs, Vg);
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+    }
+}
+
+
+template <int ORDER>
+__global__ __launch_bounds__(BLOCK) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
+    __shared__ double sh[BLOCK / 64];
+    __shared__ double lds[12 * BLOCK];
+    if (*a.stop) return;
+    const uint32_t N = m.N;
+    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
+    const bool live = c0 < N;
+    const uint32_t c = live ? c0 : N - 1;
+
+    const uint32_t code = m.fcode[c];
+    const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
+    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    float Vc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    float Vn[4][5];
+    if (ORDER != 0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t idx = w[s] & 0x1FFFFFFFu;
+            const float *src = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
+            const size_t stride = (w[s] & 0x80000000u) ? 1 : TILE;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
+        }
+    }
+    CellGeo g;
+    load_cell(m, c, g);
+    stage_nodes(lds, g);
+    V3f S[4], r[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) face_geo_mx(lds, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+    const float ivol = (float)(1.0 / g.vol);
+
+    if (ORDER == 0) {
+        if (live) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a.FV[fix(c, s, k)] = Vc[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, vmax = -FLT_MAX, vmin = FLT_MAX;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
+                const float uf = fflip(0.5f * (Vc[k] + Vn[s][k]) * ivol, mneg);
+                g0 += uf * S[s].x;
+                g1 += uf * S[s].y;
+                g2 += uf * S[s].z;
+                if (ORDER == 2) {
+                    vmax = fmaxf(vmax, fmaxf(Vc[k], Vn[s][k]));
+                    vmin = fminf(vmin, fminf(Vc[k], Vn[s][k]));
+                }
+            }
+            float delta[4];
+            float psi = (ORDER == 2) ? FLT_MAX : 1.0f;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                delta[s] = g0 * r[s].x + g1 * r[s].y + g2 * r[s].z;
+                if (ORDER == 2) psi = fminf(psi, barth_phi_f(delta[s], vmax, vmin, Vc[k]));
+            }
+            if (live) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) a.FV[fix(c, s, k)] = Vc[k] + psi * delta[s];
+                if (a.grad_aux) {
+                    a.grad_aux[15 * (size_t)c + k] = g0;
+                    a.grad_aux[15 * (size_t)c + k + 5] = g1;
+                    a.grad_aux[15 * (size_t)c + k + 10] = g2;
+                    a.psi_aux[5 * (size_t)c + k] = psi;
+                }
+            }
+        }
+    }
+    if (a.dt_part) {
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (!(w[s] & 0x80000000u)) {
+                d0 += 0.5 * fabs((double)S[s].x);
+                d1 += 0.5 * fabs((double)S[s].y);
+                d2 += 0.5 * fabs((double)S[s].z);
+            }
+        }
+        const double V0 = Vc[0], V4 = Vc[4];
+        const double cs = sqrt(Consts<double>::GAMMA * V4 / V0);
+        const double sx = (fabs((double)Vc[1]) + cs) * d0;
+        const double sy = (fabs((double)Vc[2]) + cs) * d1;
+        const double sz = (fabs((double)Vc[3]) + cs) * d2;
+        const double t = a.CFL * g.vol / ((sx + sy) + sz);
+        const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;
+        const double v = block_reduce<true>(dtc, sh);
+        if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
+    }
+}
+
+
+__global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
+    __shared__ double sh[BLOCK / 64];
+    __shared__ double lds[12 * BLOCK];
+    if (*a.stop) return;
+    const uint32_t N = m.N;
+    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
+    const bool live = c0 < N;
+    const uint32_t c = live ? c0 : N - 1;
+
+    const uint32_t code = m.fcode[c];
+    const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
+    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    float Vo4[4][5], Vn4[4][5];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
+        const bool bnd = w[s] & 0x80000000u;
+        const float *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
+        const size_t stride = bnd ? 1 : TILE;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Vo4[s][k] = a.FV[fix(c, s, k)];
+            Vn4[s][k] = src[k * stride];
+        }
+    }
+    CellGeo g;
+    load_cell(m, c, g);
+    stage_nodes(lds, g);
+    double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t byte = (code >> (8 * s)) & 0xFFu;
+        V3f S;
+        face_geo_mx(lds, g, byte, S, nullptr);
+        float UO[5], UN[5], F[5];
+        prim_to_cons_f(Vo4[s], UO);
+        prim_to_cons_f(Vn4[s], UN);
+        const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
+        rusanov_side_f(UO, UN, S, mneg, F);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) R[k] += (double)fflip(F[k], mneg ^ 0x80000000u);
+    }
+    if (live && a.check) {
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) ok = ok && (R[k] - R[k] == 0.0);
+        if (!ok) atomicAdd(a.bad_flux, 1);
+    }
+    if (a.R_aux) {
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
+        }
+    } else {
+        const double dt = *a.dt;
+        double Un[5], Vnew[5];
+        if (a.kind == 1) {
+            const double f = dt / g.vol;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Un[k] = a.U[tix(c, k)] + f * R[k];
+        } else if (a.kind == 2) {
+            const double f = 1.0 / 4.0 * dt / g.vol;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
+            }
+        } else {
+            const double f = 2.0 / 3.0 * dt / g.vol;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
+            }
+        }
+        cons_to_prim_d(Un, Vnew);
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                a.Uout[tix(c, k)] = Un[k];
+                a.Vout[tix(c, k)] = (float)Vnew[k];
+            }
+            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(a.bad_state, 1);
+        }
+    }
+    if (a.res_part) {
+        const double r2 = live ? R[0] * R[0] : 0.0;
+        const double v = block_reduce<false>(r2, sh);
+        if (threadIdx.x == 0) a.res_part[blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_prim_mx(const double *U, float *V, uint32_t N) {
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= N) return;
+    double u[5], v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = U[tix(c, k)];
+    cons_to_prim_d(u, v);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) V[tix(c, k)] = (float)v[k];
+}
+
+#endif  // NF_MIXED_KERNELS
+
+// host-side launchers (compiled in solver_mixed.hip)
+hipError_t launch_ghost_mx(bool face, dim3 g, hipStream_t st, const DevMesh &m, const GhostArgsMx &a);
+hipError_t launch_sweepA_mx(int order, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a);
+hipError_t launch_sweepB_mx(dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a);
+hipError_t launch_prim_mx(dim3 g, hipStream_t st, const double *U, float *V, uint32_t N);
+
+}  // namespace mx
+}  // namespace nf

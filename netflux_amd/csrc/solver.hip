@@ -17,6 +17,7 @@
 
 #include "../../include/netflux.h"
 #include "kernels_fp64.hpp"
+#include "kernels_mixed.hpp"
 #include "mesh.hpp"
 
 using namespace nf;
@@ -116,6 +117,9 @@ struct nf_ctx {
     uint32_t *bface = nullptr;
     uint8_t *bbc = nullptr;
     double *Gc = nullptr, *Gf = nullptr;  // ghost states [nbf][5]
+    bool mixed = false;                   // NF_PREC_MIXED: FP32 copies below
+    float *V32 = nullptr, *FV32 = nullptr, *Gc32 = nullptr, *Gf32 = nullptr;
+    FarField<float> fs32{};
     int xcd_remap = 1;
     // device arrays
     double *nodes = nullptr;
@@ -186,6 +190,23 @@ int launch(nf_ctx *ctx, const char *name, K kernel, dim3 grid, dim3 block, Args.
     return NF_OK;
 }
 
+// same as launch() for a launcher callable returning hipError_t
+template <typename F>
+int launch_fn(nf_ctx *ctx, const char *name, F fn) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        HIPC(hipEventRecord(e0, ctx->stream));
+    }
+    HIPC(fn());
+    if (ctx->timing) {
+        HIPC(hipEventRecord(e1, ctx->stream));
+        ctx->pending.push_back({name, {e0, e1}});
+    }
+    return NF_OK;
+}
+
 int flush_timing(nf_ctx *ctx) {
     HIPC(hipStreamSynchronize(ctx->stream));
     for (auto &p : ctx->pending) {
@@ -217,9 +238,15 @@ SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux) {
 
 int run_ghost(nf_ctx *ctx, bool face) {
     if (ctx->nbf == 0) return NF_OK;
+    dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
+    if (ctx->mixed) {
+        mx::GhostArgsMx a{ctx->bface, ctx->bbc, ctx->nbf, face ? ctx->FV32 : ctx->V32, face ? ctx->Gf32 : ctx->Gc32,
+                          ctx->fs32, ctx->iscal + 2};
+        return launch_fn(ctx, face ? "ghost_face_mx" : "ghost_cell_mx",
+                         [&] { return mx::launch_ghost_mx(face, g, ctx->stream, devmesh(ctx), a); });
+    }
     GhostArgs a{ctx->bface, ctx->bbc, ctx->nbf, face ? ctx->FV : ctx->V, face ? ctx->Gf : ctx->Gc, ctx->fs,
                 ctx->iscal + 2};
-    dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
     if (face) return launch(ctx, "ghost_face", k_ghost<true>, g, b, devmesh(ctx), a);
     return launch(ctx, "ghost_cell", k_ghost<false>, g, b, devmesh(ctx), a);
 }
@@ -227,9 +254,17 @@ int run_ghost(nf_ctx *ctx, bool face) {
 int run_sweepA(nf_ctx *ctx, bool dt, bool aux) {
     int rc;
     if ((rc = run_ghost(ctx, false))) return rc;
-    SweepAArgs a = argsA(ctx, dt, aux);
     const DevMesh m = devmesh(ctx);
     dim3 g(ctx->nb), b(BLOCK);
+    if (ctx->mixed) {
+        mx::SweepAArgsMx x{ctx->V32, ctx->Gc32, ctx->FV32, dt ? ctx->dt_part : nullptr, ctx->prm.CFL,
+                           ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->grad_aux : nullptr,
+                           aux ? ctx->psi_aux : nullptr};
+        const int order = ctx->prm.spatial_order == NF_FIRST_ORDER ? 0 : (ctx->prm.limiter == NF_LIMITER_NONE ? 1 : 2);
+        return launch_fn(ctx, order == 2 ? "sweepA_barth_mx" : "sweepA_mx",
+                         [&] { return mx::launch_sweepA_mx(order, g, ctx->stream, m, x); });
+    }
+    SweepAArgs a = argsA(ctx, dt, aux);
     if (ctx->prm.spatial_order == NF_FIRST_ORDER)
         return launch(ctx, "sweepA_first", k_sweepA<0>, g, b, m, a);
     if (ctx->prm.limiter == NF_LIMITER_NONE)
@@ -241,6 +276,13 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double 
                bool aux) {
     int rc;
     if ((rc = run_ghost(ctx, true))) return rc;
+    if (ctx->mixed) {
+        mx::SweepBArgsMx x{ctx->FV32, ctx->Gf32, U, Uo, Uout, ctx->V32, ctx->scal, kind,
+                           res ? ctx->res_part : nullptr, ctx->iscal + 4, ctx->prm.check_physical_validity,
+                           ctx->iscal + 5, ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+        return launch_fn(ctx, "sweepB_mx",
+                         [&] { return mx::launch_sweepB_mx(dim3(ctx->nb), ctx->stream, devmesh(ctx), x); });
+    }
     SweepBArgs a{};
     a.FV = ctx->FV;
     a.Gf = ctx->Gf;
@@ -379,6 +421,17 @@ int download_tiled(nf_ctx *ctx, const double *dev, double *aos) {
     return NF_OK;
 }
 
+int download_tiled_f32(nf_ctx *ctx, const float *dev, double *aos) {
+    const uint32_t N = ctx->N;
+    std::vector<float> d((size_t)5 * ctx->npad);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < N; ++c)
+        for (int k = 0; k < 5; ++k)
+            aos[5 * (size_t)ctx->perm[c] + k] = d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)];
+    return NF_OK;
+}
+
 // device-order AoS [N][rows] -> reference-order AoS
 int download_aos(nf_ctx *ctx, const double *dev, int rows, double *aos) {
     const uint32_t N = ctx->N;
@@ -391,9 +444,13 @@ int download_aos(nf_ctx *ctx, const double *dev, int rows, double *aos) {
 }
 
 int after_state_set(nf_ctx *ctx) {
-    hipLaunchKernelGGL(k_prim, dim3((ctx->N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua,
-                       ctx->V, ctx->N);
-    HIPC(hipGetLastError());
+    if (ctx->mixed) {
+        HIPC(mx::launch_prim_mx(dim3((ctx->N + BLOCK - 1) / BLOCK), ctx->stream, ctx->Ua, ctx->V32, ctx->N));
+    } else {
+        hipLaunchKernelGGL(k_prim, dim3((ctx->N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua,
+                           ctx->V, ctx->N);
+        HIPC(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->iscal, 0, 8, ctx->scal, 0, 0.0, 1);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(ctx->stream));
@@ -412,7 +469,7 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     *out = nullptr;
     if (prm->inv_flux_scheme != NF_FLUX_RUSANOV) return NF_E_ARG;  // HLLC: assert(false) in the reference
     if (prm->time_scheme != NF_TVD_RK3 && prm->time_scheme != NF_EXPLICIT_EULER) return NF_E_ARG;
-    if (prm->precision != NF_PREC_FP64) return NF_E_ARG;
+    if (prm->precision != NF_PREC_FP64 && prm->precision != NF_PREC_MIXED) return NF_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NF_E_NOGPU;
     std::unique_ptr<nf_ctx> hold(new nf_ctx);
@@ -443,6 +500,8 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     ctx->perm = L.perm;
     ctx->iperm = L.iperm;
     ctx->fs = make_farfield<double>(prm->primvars_inf);
+    ctx->fs32 = make_farfield<float>(prm->primvars_inf);
+    ctx->mixed = prm->precision == NF_PREC_MIXED;
     ctx->nb = (int)((g.N + BLOCK - 1) / BLOCK);
     ctx->npad = (g.N + TILE - 1) / TILE * TILE;
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -455,13 +514,23 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
     const size_t NP = ctx->npad;
     if ((rc = dalloc(ctx, &ctx->Ua, 5 * NP))) return rc;
     if ((rc = dalloc(ctx, &ctx->Ub, 5 * NP))) return rc;
-    if ((rc = dalloc(ctx, &ctx->V, 5 * NP))) return rc;
-    if ((rc = dalloc(ctx, &ctx->FV, 20 * NP))) return rc;
+    if (ctx->mixed) {
+        if ((rc = dalloc(ctx, &ctx->V32, 5 * NP))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV32, 20 * NP))) return rc;
+    } else {
+        if ((rc = dalloc(ctx, &ctx->V, 5 * NP))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV, 20 * NP))) return rc;
+    }
     ctx->nbf = (uint32_t)L.bface.size();
     if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
-    if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
-    if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+    if (ctx->mixed) {
+        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
+    } else {
+        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+    }
     if (ctx->nbf) {
         HIPC(hipMemcpy(ctx->bface, L.bface.data(), L.bface.size() * 4, hipMemcpyHostToDevice));
         HIPC(hipMemcpy(ctx->bbc, L.bbc.data(), L.bbc.size(), hipMemcpyHostToDevice));
@@ -517,9 +586,11 @@ int nf_set_freestream(nf_ctx *ctx) {
     if (rc) return rc;
     if ((rc = after_state_set(ctx))) return rc;
     // primvars = V_inf exactly (set_freestream_values, src/SolverData.cpp:50-58)
-    for (uint32_t c = 0; c < ctx->N; ++c)
-        for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = V[k];
-    if ((rc = upload_aos(ctx, aos.data(), ctx->V))) return rc;
+    if (!ctx->mixed) {
+        for (uint32_t c = 0; c < ctx->N; ++c)
+            for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = V[k];
+        if ((rc = upload_aos(ctx, aos.data(), ctx->V))) return rc;
+    }
     ctx->uold_zero = true;
     return NF_OK;
 }
@@ -705,8 +776,12 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
     }
     if ((rc = clear_flags(ctx))) return rc;
     // primvars of the current state
-    hipLaunchKernelGGL(k_prim, dim3((N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua, ctx->V, N);
-    HIPC(hipGetLastError());
+    if (ctx->mixed) {
+        HIPC(mx::launch_prim_mx(dim3((N + BLOCK - 1) / BLOCK), ctx->stream, ctx->Ua, ctx->V32, N));
+    } else {
+        hipLaunchKernelGGL(k_prim, dim3((N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua, ctx->V, N);
+        HIPC(hipGetLastError());
+    }
     if ((rc = run_sweepA(ctx, false, true))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, ctx->Ub, false, true))) return rc;
     ctx->aux_valid = true;
@@ -717,6 +792,7 @@ int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
     if (!ctx || !out) return NF_E_ARG;
     switch (which) {
     case NF_FIELD_PRIM:
+        if (ctx->mixed) return download_tiled_f32(ctx, ctx->V32, out);
         return download_tiled(ctx, ctx->V, out);
     case NF_FIELD_STATE_OLD: {
         if (ctx->uold_zero) {

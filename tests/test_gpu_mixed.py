@@ -1,0 +1,56 @@
+"""Mixed precision (FP32 gradient/limiter/flux, FP64 state/RK/norms) against
+the FP64 oracle.  The error bound is stated in each test: FP32 face
+arithmetic (unit round-off 6e-8) accumulated over the steps run."""
+import numpy as np
+import pytest
+
+import netflux_amd as nf
+from tests.cases import pulse_config, pulse_state, rel_l2, sod_config, sod_state
+from tests.oracle_bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(mesh, cfg, U0, steps, uold_equals_u=True):
+    print(f"[case] mixed cells={mesh.n_cells} steps={steps}", flush=True)
+    cfg.rk3_zero_old_first_step = not uold_equals_u
+    g = nf.EulerSolver(cfg, mesh)
+    g.set_state(U0, U0 if uold_equals_u else None)
+    st = g.run(steps, use_graph=True, check_every=10)
+    return g, st
+
+
+def test_mixed_sod_error_vs_fp64_oracle():
+    mesh = nf.Mesh.box(16)
+    U0 = sod_state(mesh)
+    g, _ = _run(mesh, sod_config(precision="mixed"), U0, 50)
+    ora = Oracle.from_mesh(mesh, sod_config())
+    ora.set_state(U0, uold_equals_u=True)
+    ora.run(50)
+    Ug, Uo = g.get_solver_data().get_solution(), ora.get("U")
+    errs = [rel_l2(Ug[:, k], Uo[:, k]) if np.linalg.norm(Uo[:, k]) > 0 else float(np.abs(Ug[:, k]).max())
+            for k in range(5)]
+    print("per-variable relative L2 vs FP64 oracle:", errs)
+    assert rel_l2(Ug, Uo) < 2e-5  # FP32 flux/gradient/limiter, 150 stages
+    assert errs[0] < 2e-5 and errs[4] < 2e-5
+
+
+def test_mixed_conserves_mass_and_energy():
+    mesh = nf.Mesh.box(10, jitter=0.15, seed=12)
+    U0 = sod_state(mesh)
+    g, _ = _run(mesh, sod_config(precision="mixed"), U0, 40)
+    vol = mesh.fv_grid()["volume"][: mesh.n_cells]
+    U = g.get_solver_data().get_solution()
+    # each face flux is evaluated bit-identically by both cells and summed in FP64
+    assert abs(vol @ U[:, 0] - vol @ U0[:, 0]) < 1e-13 * (vol @ U0[:, 0])
+    assert abs(vol @ U[:, 4] - vol @ U0[:, 4]) < 1e-13 * (vol @ U0[:, 4])
+
+
+def test_mixed_farfield_pulse():
+    mesh = nf.Mesh.box(12, jitter=0.15, seed=5, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    g, _ = _run(mesh, pulse_config(precision="mixed"), U0, 30)
+    ora = Oracle.from_mesh(mesh, pulse_config())
+    ora.set_state(U0, uold_equals_u=True)
+    ora.run(30)
+    assert rel_l2(g.get_solver_data().get_solution(), ora.get("U")) < 1e-5
