@@ -74,6 +74,9 @@ typedef struct nf_params {
     int32_t rk3_zero_old_first_step; /* 1 = reference behaviour            */
     int32_t precision;            /* NF_PREC_*                             */
     int32_t cell_order;           /* NF_ORDER_*                            */
+    /* 1: each cell advances with its own CFL step (steady-state runs).  The
+     * reference always takes the global minimum (src/Solver.cpp:252-265). */
+    int32_t local_time_step;
 } nf_params;
 
 /* ---- host mesh utilities (CPU only) ------------------------------------ */

@@ -42,6 +42,7 @@ class Params(C.Structure):
         ("check_physical_validity", C.c_int32),
         ("rk3_zero_old_first_step", C.c_int32),
         ("precision", C.c_int32), ("cell_order", C.c_int32),
+        ("local_time_step", C.c_int32),
     ]
 
 

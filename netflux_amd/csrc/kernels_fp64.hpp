@@ -364,6 +364,7 @@ struct SweepAArgs {
     int xcd_remap;
     double *grad_aux;  // [N][15] column-major 5x3 per cell, or null
     double *psi_aux;   // [N][5] or null
+    double *dt_cell;   // [N] per-cell CFL step (local time stepping), or null
 };
 
 // ORDER: 0 first order, 1 second order without limiter, 2 second order Barth
@@ -478,6 +479,7 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepA(DevMesh m, Swe
         const double sz = (fabs(Vc[3]) + cs) * d2;
         const double t = a.CFL * g.vol / ((sx + sy) + sz);
         const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;  // NaN/+inf never win std::min
+        if (a.dt_cell && live) a.dt_cell[c] = t;
         const double v = block_reduce<true>(dtc, sh);
         if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
     }
@@ -500,6 +502,7 @@ struct SweepBArgs {
     const int *stop;
     int xcd_remap;
     double *R_aux;     // [N][5] or null: store R, skip the update
+    const double *dt_cell;  // per-cell step (local time stepping), or null
 };
 
 __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, SweepBArgs a) {
@@ -587,7 +590,7 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
             for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
         }
     } else {
-        const double dt = *a.dt;
+        const double dt = a.dt_cell ? a.dt_cell[c] : *a.dt;
         double Un[5], Vnew[5];
         // Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:98-132)
         if (a.kind == 1) {

@@ -35,6 +35,7 @@ struct SweepAArgsMx {
     const int *stop;
     int xcd_remap;
     double *grad_aux, *psi_aux;
+    double *dt_cell;
 };
 
 struct SweepBArgsMx {
@@ -53,6 +54,7 @@ struct SweepBArgsMx {
     const int *stop;
     int xcd_remap;
     double *R_aux;
+    const double *dt_cell;
 };
 
 #ifdef NF_MIXED_KERNELS
@@ -238,6 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) 
         const double sz = (fabs((double)Vc[3]) + cs) * d2;
         const double t = a.CFL * g.vol / ((sx + sy) + sz);
         const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;
+        if (a.dt_cell && live) a.dt_cell[c] = t;
         const double v = block_reduce<true>(dtc, sh);
         if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
     }
@@ -298,7 +301,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
             for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
         }
     } else {
-        const double dt = *a.dt;
+        const double dt = a.dt_cell ? a.dt_cell[c] : *a.dt;
         double Un[5], Vnew[5];
         if (a.kind == 1) {
             const double f = dt / g.vol;

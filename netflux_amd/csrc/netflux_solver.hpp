@@ -41,6 +41,7 @@ struct Config {
     bool rk3_zero_old_first_step = true;
     int precision = NF_PREC_FP64;
     int cell_order = NF_ORDER_MORTON;
+    bool local_time_step = false;  // per-cell CFL step for steady runs (not the reference)
     // run state (Config::delta_time / time / timestep)
     double delta_time = 0.0;
     double time = 0.0;
@@ -66,6 +67,7 @@ struct Config {
         p.rk3_zero_old_first_step = rk3_zero_old_first_step;
         p.precision = precision;
         p.cell_order = cell_order;
+        p.local_time_step = local_time_step;
         return p;
     }
 };

@@ -119,6 +119,7 @@ struct nf_ctx {
     double *Gc = nullptr, *Gf = nullptr;  // ghost states [nbf][5]
     bool mixed = false;                   // NF_PREC_MIXED: FP32 copies below
     float *V32 = nullptr, *FV32 = nullptr, *Gc32 = nullptr, *Gf32 = nullptr;
+    double *dt_cell = nullptr;            // local time stepping: per-cell step
     FarField<float> fs32{};
     int xcd_remap = 1;
     // device arrays
@@ -222,7 +223,7 @@ int flush_timing(nf_ctx *ctx) {
     return NF_OK;
 }
 
-SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux) {
+SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux, bool dt_cell) {
     SweepAArgs a{};
     a.V = ctx->V;
     a.Gc = ctx->Gc;
@@ -233,6 +234,7 @@ SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux) {
     a.xcd_remap = ctx->xcd_remap;
     a.grad_aux = aux ? ctx->grad_aux : nullptr;
     a.psi_aux = aux ? ctx->psi_aux : nullptr;
+    a.dt_cell = dt_cell ? ctx->dt_cell : nullptr;
     return a;
 }
 
@@ -251,7 +253,7 @@ int run_ghost(nf_ctx *ctx, bool face) {
     return launch(ctx, "ghost_cell", k_ghost<false>, g, b, devmesh(ctx), a);
 }
 
-int run_sweepA(nf_ctx *ctx, bool dt, bool aux) {
+int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dt_cell = false) {
     int rc;
     if ((rc = run_ghost(ctx, false))) return rc;
     const DevMesh m = devmesh(ctx);
@@ -259,12 +261,12 @@ int run_sweepA(nf_ctx *ctx, bool dt, bool aux) {
     if (ctx->mixed) {
         mx::SweepAArgsMx x{ctx->V32, ctx->Gc32, ctx->FV32, dt ? ctx->dt_part : nullptr, ctx->prm.CFL,
                            ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->grad_aux : nullptr,
-                           aux ? ctx->psi_aux : nullptr};
+                           aux ? ctx->psi_aux : nullptr, dt_cell ? ctx->dt_cell : nullptr};
         const int order = ctx->prm.spatial_order == NF_FIRST_ORDER ? 0 : (ctx->prm.limiter == NF_LIMITER_NONE ? 1 : 2);
         return launch_fn(ctx, order == 2 ? "sweepA_barth_mx" : "sweepA_mx",
                          [&] { return mx::launch_sweepA_mx(order, g, ctx->stream, m, x); });
     }
-    SweepAArgs a = argsA(ctx, dt, aux);
+    SweepAArgs a = argsA(ctx, dt, aux, dt_cell);
     if (ctx->prm.spatial_order == NF_FIRST_ORDER)
         return launch(ctx, "sweepA_first", k_sweepA<0>, g, b, m, a);
     if (ctx->prm.limiter == NF_LIMITER_NONE)
@@ -279,7 +281,8 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double 
     if (ctx->mixed) {
         mx::SweepBArgsMx x{ctx->FV32, ctx->Gf32, U, Uo, Uout, ctx->V32, ctx->scal, kind,
                            res ? ctx->res_part : nullptr, ctx->iscal + 4, ctx->prm.check_physical_validity,
-                           ctx->iscal + 5, ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+                           ctx->iscal + 5, ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->R_aux : nullptr,
+                           ctx->prm.local_time_step ? ctx->dt_cell : nullptr};
         return launch_fn(ctx, "sweepB_mx",
                          [&] { return mx::launch_sweepB_mx(dim3(ctx->nb), ctx->stream, devmesh(ctx), x); });
     }
@@ -299,6 +302,7 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double 
     a.stop = ctx->iscal + 2;
     a.xcd_remap = ctx->xcd_remap;
     a.R_aux = aux ? ctx->R_aux : nullptr;
+    a.dt_cell = ctx->prm.local_time_step ? ctx->dt_cell : nullptr;
     return launch(ctx, "sweepB", k_sweepB, dim3(ctx->nb), dim3(BLOCK), devmesh(ctx), a);
 }
 
@@ -310,8 +314,10 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double 
 int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
     int rc;
     const bool rk3 = ctx->prm.time_scheme == NF_TVD_RK3;
+    const bool lts = ctx->prm.local_time_step;
     int *is = ctx->iscal;
-    if ((rc = run_sweepA(ctx, !rk3, false))) return rc;
+    // local time stepping: per-cell steps from the state entering the step
+    if ((rc = run_sweepA(ctx, !rk3 || lts, false, lts))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, rk3 ? ctx->Ub : ctx->Ua, true, false))) return rc;
     if ((rc = launch(ctx, "reduce_res", k_reduce<false>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->res_part,
                      ctx->nb, ctx->red2 + ctx->nb2, (const int *)(is + 2))))
@@ -324,7 +330,7 @@ int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
         const double *Uo = zero_old ? nullptr : ctx->Ua;
         if ((rc = run_sweepA(ctx, false, false))) return rc;
         if ((rc = run_sweepB(ctx, 2, ctx->Ub, Uo, ctx->Ub, false, false))) return rc;
-        if ((rc = run_sweepA(ctx, true, false))) return rc;
+        if ((rc = run_sweepA(ctx, !lts, false))) return rc;
         if ((rc = run_sweepB(ctx, 3, ctx->Ub, Uo, ctx->Ua, false, false))) return rc;
     }
     if ((rc = launch(ctx, "reduce_dt", k_reduce<true>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->dt_part,
@@ -521,6 +527,7 @@ int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
         if ((rc = dalloc(ctx, &ctx->V, 5 * NP))) return rc;
         if ((rc = dalloc(ctx, &ctx->FV, 20 * NP))) return rc;
     }
+    if (prm->local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NP))) return rc;
     ctx->nbf = (uint32_t)L.bface.size();
     if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;

@@ -62,6 +62,7 @@ class Config:
     rk3_zero_old_first_step: bool = True
     precision: str = "fp64"               # fp64 | mixed
     cell_order: str = "morton"            # input | rcm | morton (device cell numbering)
+    local_time_step: bool = False         # per-cell CFL step (steady runs; not the reference)
     # run-time state held by Config in the reference
     delta_time: float = 0.0
     time: float = 0.0
@@ -84,6 +85,7 @@ class Config:
         p.precision = {"fp64": A.NF_PREC_FP64, "mixed": A.NF_PREC_MIXED}[self.precision]
         p.cell_order = {"input": A.NF_ORDER_INPUT, "rcm": A.NF_ORDER_RCM,
                         "morton": A.NF_ORDER_MORTON}[self.cell_order]
+        p.local_time_step = int(self.local_time_step)
         return p
 
 

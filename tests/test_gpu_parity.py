@@ -153,7 +153,8 @@ def test_residual_history_matches_oracle():
     assert len(res) == 5
     expect = []
     for _ in range(5):
+        dt = ora.calc_timestep()  # from the primvars as last set (stale), before R(U^n)
         ora.eval_flux_balance()
         expect.append(np.linalg.norm(ora.get("R")[:, 0]))
-        ora.step(ora.calc_timestep())
+        ora.step(dt)
     np.testing.assert_allclose(res, expect, rtol=1e-13)
