@@ -191,6 +191,37 @@ int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out,
 /* Sizes of a context (cells, faces) for callers sizing their buffers. */
 int nf_ctx_sizes(const nf_ctx *ctx, nf_fv_sizes *sizes);
 
+/* ---- batches: several meshes in one context -------------------------------
+ * No counterpart in the reference, whose Driver holds one solver
+ * (src/Driver.cpp:36 asserts solvers.size() == 1).  All instances share the
+ * options of params[0] except the freestream (primvars_inf), which is per
+ * instance; each instance has its own time step (global per instance, or
+ * local), residual history, first-step flag and convergence state, and every
+ * sweep is one launch over all instances.  The single-mesh functions above
+ * act on a context of one instance. */
+int nf_create_batch(int32_t n, const nf_mesh_desc *meshes, const nf_params *params, nf_ctx **out);
+int nf_ctx_instances(const nf_ctx *ctx, int32_t *n);
+int nf_inst_sizes(const nf_ctx *ctx, int32_t inst, nf_fv_sizes *sizes);
+int nf_inst_set_state(nf_ctx *ctx, int32_t inst, const double *U, const double *U_old_or_null);
+int nf_inst_get_state(nf_ctx *ctx, int32_t inst, double *U);
+int nf_inst_residuals(nf_ctx *ctx, int32_t inst, double *out, int32_t max, int32_t *n_out);
+/* Runs every instance until it converges (residual_drop) or n_steps steps
+ * are done; stats[n] per instance, elapsed_ms is the batch wall time. */
+int nf_run_batch(nf_ctx *ctx, const nf_run_opts *opts, nf_run_stats *stats);
+
+/* Autotuner: times `steps` steps for each launch configuration (threads per
+ * block 128 or 256, XCD-contiguous block order on or off) from a copy of the
+ * current state, restores the state and keeps the fastest configuration. */
+typedef struct nf_tune_result {
+    int32_t block;
+    int32_t xcd_remap;
+    double ms_per_step;
+    int32_t n_tried;
+    double tried_ms[8];           /* per candidate, in the order tried      */
+} nf_tune_result;
+int nf_autotune(nf_ctx *ctx, int32_t steps, nf_tune_result *out);
+int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,4 +5,4 @@ HIP kernels for gfx950 behind the C ABI of include/netflux.h
 mirror of the reference's Config / EulerSolver / SolverData / Driver.
 """
 from ._abi import LIB_PATH, lib  # noqa: F401
-from .solver import Config, Driver, EulerSolver, Mesh, SolverData  # noqa: F401
+from .solver import BatchSolver, Config, Driver, EulerSolver, Mesh, SolverData  # noqa: F401

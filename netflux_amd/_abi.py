@@ -68,6 +68,11 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("avg_ms", C.c_double), ("calls", C.c_int32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_int32), ("xcd_remap", C.c_int32), ("ms_per_step", C.c_double),
+                ("n_tried", C.c_int32), ("tried_ms", C.c_double * 8)]
+
+
 # (name, restype, argtypes) of every symbol declared in include/netflux.h
 SIGNATURES = [
     ("nf_mesh_box", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
@@ -92,6 +97,16 @@ SIGNATURES = [
     ("nf_get_field", C.c_int, [C.c_void_p, C.c_int32, f64p]),
     ("nf_time_kernels", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelTime), C.c_int32, i32p]),
     ("nf_ctx_sizes", C.c_int, [C.c_void_p, C.POINTER(FVSizes)]),
+    ("nf_create_batch", C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Params),
+                                  C.POINTER(C.c_void_p)]),
+    ("nf_ctx_instances", C.c_int, [C.c_void_p, i32p]),
+    ("nf_inst_sizes", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(FVSizes)]),
+    ("nf_inst_set_state", C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
+    ("nf_inst_get_state", C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    ("nf_inst_residuals", C.c_int, [C.c_void_p, C.c_int32, f64p, C.c_int32, i32p]),
+    ("nf_run_batch", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+    ("nf_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(TuneResult)]),
+    ("nf_set_launch", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
 ]
 
 _lib = None

@@ -20,6 +20,11 @@
 //   gradient and min/max) and k_ghost<true> (ghost of the reconstructed face
 //   state, for the boundary flux), so the sweeps carry no boundary code.
 //
+// Several meshes (instances) can share one launch: their cells are
+// concatenated, each instance padded to whole blocks of 256 cells, and every
+// block looks up its instance for the time step, the first-step flag, the
+// stop flag and the error counters (InstState).
+//
 // Layouts: cell fields are tiled by 64 cells (one wavefront):
 //   V, U  : [tile][5][64]          FV : [tile][4 slots][5][64]
 // so a wave's own-cell loads and stores are fully coalesced; cnodes and nbr
@@ -33,10 +38,7 @@
 
 namespace nf {
 
-constexpr int BLOCK = 256;  // threads = cells per block
-#ifndef NF_SWEEP_WAVES
-#define NF_SWEEP_WAVES 1   // minimum waves per SIMD requested for the sweeps
-#endif
+constexpr int BLOCK = 256;  // threads of the auxiliary kernels; instance padding unit
 constexpr int TILE = 64;    // cells per layout tile
 
 __device__ __forceinline__ size_t tix(uint32_t c, int k) {  // [tile][5][64]
@@ -46,13 +48,29 @@ __device__ __forceinline__ size_t fix(uint32_t c, uint32_t s, int k) {  // [tile
     return (((size_t)(c >> 6) * 4 + s) * 5 + k) * TILE + (c & 63u);
 }
 
+// Per-instance run state, one record per mesh of the context.
+struct InstState {
+    double dt;         // time step of the current step
+    double time;       // sum of completed steps' dt
+    int nstep;         // completed steps
+    int err;           // NF_E_INVALID_DT once raised
+    int stop;          // nonzero: kernels skip this instance
+    int converged;     // residual drop reached
+    int bad_flux;      // cells with non-finite flux balance (checked runs)
+    int bad_state;     // cells with rho <= 0, p <= 0 or NaN after an update
+    int uold_zero;     // U_old is all zeros (before the first step)
+    int pad;
+};
+
 struct DevMesh {
-    uint32_t N;
-    double three, six;  // 3.0 and 6.0, divisors of the centroid / volume formulas
+    double three, six;       // 3.0 and 6.0, divisors of the centroid / volume formulas
     const double *nodes;
     const uint32_t *cnodes;
     const uint32_t *nbr;
     const uint32_t *fcode;
+    const uint16_t *blk_inst;  // [blocks] instance of each 256-cell block
+    const uint32_t *inst_end;  // [instances] one past the instance's last device cell
+    InstState *st;             // [instances]
 };
 
 using V3d = Vec3T<double>;
@@ -62,15 +80,6 @@ __device__ __forceinline__ V3d vadd(V3d a, V3d b) { return {a.x + b.x, a.y + b.y
 __device__ __forceinline__ V3d vscl(double s, V3d a) { return {s * a.x, s * a.y, s * a.z}; }
 __device__ __forceinline__ V3d vcross(V3d a, V3d b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-// node q of the cell, selected per scalar component so the choice stays in
-// registers (v_cndmask) instead of becoming an indexed scratch access
-__device__ __forceinline__ double dsel(uint32_t q, double a0, double a1, double a2, double a3) {
-    double r = a0;
-    r = (q == 1u) ? a1 : r;
-    r = (q == 2u) ? a2 : r;
-    r = (q == 3u) ? a3 : r;
-    return r;
 }
 
 // IEEE maxNum/minNum (v_max_f64 / v_min_f64): over finite values the
@@ -221,6 +230,22 @@ __device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t nb, int x
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// The lane's cell, its instance, and whether it is a real cell.  Dead lanes
+// of the last block of an instance mirror the instance's last cell and store
+// nothing.
+struct LaneCell {
+    uint32_t c, inst, lb;  // cell, instance, logical block
+    bool live;
+};
+template <int B>
+__device__ __forceinline__ LaneCell lane_cell(const DevMesh &m, int xcd_remap) {
+    const uint32_t lb = logical_block(blockIdx.x, gridDim.x, xcd_remap);
+    const uint32_t c0 = lb * B + threadIdx.x;
+    const uint32_t inst = m.blk_inst[(lb * B) / BLOCK];
+    const uint32_t end = m.inst_end[inst];
+    return {c0 < end ? c0 : end - 1, inst, lb, c0 < end};
+}
+
 struct CellGeo {
     V3d p0, p1, p2, p3;
     V3d cen;
@@ -245,52 +270,36 @@ __device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo 
     g.ivol = rcp_of(g.vol);
 }
 
-// Face normal oriented i->j (triangle formula of src/geometry/
-// GeometryUtilities.cpp:318-324, flip of src/geometry/FV_Grid.cpp:247-250)
-// and centroid-to-face vector from the face element's node order.
-__device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, uint32_t byte, V3d &S, V3d *r) {
-    const uint32_t ia = byte & 3u, ib = (byte >> 2) & 3u, ic = (byte >> 4) & 3u;
-    const V3d a = {dsel(ia, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ia, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
-                   dsel(ia, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
-    const V3d b = {dsel(ib, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ib, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
-                   dsel(ib, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
-    const V3d c = {dsel(ic, g.p0.x, g.p1.x, g.p2.x, g.p3.x), dsel(ic, g.p0.y, g.p1.y, g.p2.y, g.p3.y),
-                   dsel(ic, g.p0.z, g.p1.z, g.p2.z, g.p3.z)};
-    S = vscl(0.5, vcross(vsub(b, a), vsub(c, a)));
-    if (byte & 0x40u) S = vscl(-1, S);
-    if (r) {
-        // triangle_calc_centroid (GeometryUtilities.cpp:326-329)
-        const V3d s = vadd(vadd(a, b), c);
-        const Rcp i3 = rcp_of(m.three);
-        const V3d fc = {rdiv(s.x, i3), rdiv(s.y, i3), rdiv(s.z, i3)};
-        *r = vsub(fc, g.cen);
-    }
-}
-
-#ifndef NF_LDS_NODES
-#define NF_LDS_NODES 1
-#endif
-// The four nodes of each lane's cell staged in LDS, [12][BLOCK] (node q,
+// The four nodes of each lane's cell staged in LDS, [12][B] (node q,
 // coordinate d at row 3q+d); every lane reads only its own column, so no
 // barrier is needed, and a wave's reads of one row hit 64 distinct banks.
+// Selecting a face's nodes by index from LDS replaces register selects.
+template <int B>
 __device__ __forceinline__ void stage_nodes(double *lds, const CellGeo &g) {
     const uint32_t t = threadIdx.x;
     const V3d P[4] = {g.p0, g.p1, g.p2, g.p3};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        lds[(3 * q + 0) * BLOCK + t] = P[q].x;
-        lds[(3 * q + 1) * BLOCK + t] = P[q].y;
-        lds[(3 * q + 2) * BLOCK + t] = P[q].z;
+        lds[(3 * q + 0) * B + t] = P[q].x;
+        lds[(3 * q + 1) * B + t] = P[q].y;
+        lds[(3 * q + 2) * B + t] = P[q].z;
     }
 }
+template <int B>
 __device__ __forceinline__ V3d lds_node(const double *lds, uint32_t q) {
     const uint32_t t = threadIdx.x;
-    return {lds[(3 * q + 0) * BLOCK + t], lds[(3 * q + 1) * BLOCK + t], lds[(3 * q + 2) * BLOCK + t]};
+    return {lds[(3 * q + 0) * B + t], lds[(3 * q + 1) * B + t], lds[(3 * q + 2) * B + t]};
 }
-// face_geo with the face nodes read from the staged LDS copy
-__device__ __forceinline__ void face_geo_lds(const DevMesh &m, const CellGeo &g, const double *lds, uint32_t byte,
-                                             V3d &S, V3d *r) {
-    const V3d a = lds_node(lds, byte & 3u), b = lds_node(lds, (byte >> 2) & 3u), c = lds_node(lds, (byte >> 4) & 3u);
+
+// Face normal oriented i->j (triangle formula of src/geometry/
+// GeometryUtilities.cpp:318-324, flip of src/geometry/FV_Grid.cpp:247-250)
+// and centroid-to-face vector (triangle_calc_centroid, :326-329) from the
+// face element's node order.
+template <int B>
+__device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, const double *lds, uint32_t byte,
+                                         V3d &S, V3d *r) {
+    const V3d a = lds_node<B>(lds, byte & 3u), b = lds_node<B>(lds, (byte >> 2) & 3u),
+              c = lds_node<B>(lds, (byte >> 4) & 3u);
     S = vscl(0.5, vcross(vsub(b, a), vsub(c, a)));
     if (byte & 0x40u) S = vscl(-1, S);
     if (r) {
@@ -302,7 +311,7 @@ __device__ __forceinline__ void face_geo_lds(const DevMesh &m, const CellGeo &g,
 }
 
 // wave + block min / sum over all threads; result valid in thread 0
-template <bool MIN>
+template <bool MIN, int B>
 __device__ __forceinline__ double block_reduce(double v, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -314,22 +323,41 @@ __device__ __forceinline__ double block_reduce(double v, double *sh) {
     __syncthreads();
     if (threadIdx.x == 0) {
         v = sh[0];
-        for (int q = 1; q < BLOCK / 64; ++q) v = MIN ? ((sh[q] < v) ? sh[q] : v) : v + sh[q];
+        for (int q = 1; q < B / 64; ++q) v = MIN ? ((sh[q] < v) ? sh[q] : v) : v + sh[q];
     }
     return v;
 }
 
+// CFL step of a cell (src/Solver.cpp:254-266) with Delta S over its interior
+// faces (src/Solver.cpp:287-297), in slot order
+__device__ __forceinline__ double cell_dt(const double *Vc, const V3d *S, const uint32_t *w, double vol, double CFL) {
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (!(w[s] & 0x80000000u)) {
+            d0 += 0.5 * fabs(S[s].x);
+            d1 += 0.5 * fabs(S[s].y);
+            d2 += 0.5 * fabs(S[s].z);
+        }
+    }
+    const double cs = sound_speed_prim(Vc);
+    const double sx = (fabs(Vc[1]) + cs) * d0;
+    const double sy = (fabs(Vc[2]) + cs) * d1;
+    const double sz = (fabs(Vc[3]) + cs) * d2;
+    return CFL * vol / ((sx + sy) + sz);
+}
+
 #ifndef NF_DEVICE_HELPERS_ONLY
 // ---------------------------------------------------------------------------
-// boundary ghost states.  bface[b] = device cell | slot << 29, bbc[b] = type.
+// boundary ghost states.  bface[b] = device cell | slot << 29.
 struct GhostArgs {
     const uint32_t *bface;
-    const uint8_t *bbc;
-    uint32_t nb;         // boundary faces
-    const double *src;   // V [tile][5][64] (cell) or FV [tile][4][5][64] (face)
-    double *out;         // [nb][5]
-    FarField<double> fs;
-    const int *stop;
+    const uint8_t *bbc;          // boundary type
+    const uint16_t *binst;       // instance
+    uint32_t nb;                 // boundary faces
+    const double *src;           // V [tile][5][64] (cell) or FV [tile][4][5][64] (face)
+    double *out;                 // [nb][5]
+    const FarField<double> *fs;  // [instances]
 };
 
 // FACE = false: ghost of the cell-centre state (set_constant_ghost_values,
@@ -337,20 +365,25 @@ struct GhostArgs {
 // (src/Solver.cpp:210-212)
 template <bool FACE>
 __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
-    if (*a.stop) return;
-    const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
-    if (b >= a.nb) return;
+    __shared__ double lds[12 * BLOCK];
+    const uint32_t b0 = blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = b0 < a.nb;
+    const uint32_t b = live ? b0 : a.nb - 1;
+    const uint32_t inst = a.binst[b];
     const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
     CellGeo g;
     load_cell(m, c, g);
+    stage_nodes<BLOCK>(lds, g);
     V3d S;
-    face_geo(m, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    face_geo<BLOCK>(m, g, lds, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
     double Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
-    ghost_d((int)a.bbc[b], Vd, S, a.fs, Vg);
+    ghost_d((int)a.bbc[b], Vd, S, a.fs[inst], Vg);
+    if (live && !m.st[inst].stop) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+        for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -359,23 +392,22 @@ struct SweepAArgs {
     const double *Gc;  // [nb][5] ghost of the cell-centre state
     double *FV;        // [tile][4][5][64]
     double *dt_part;   // per-block CFL dt minimum, or null
+    double *dt_cell;   // [N] per-cell CFL step (local time stepping), or null
     double CFL;
-    const int *stop;
     int xcd_remap;
     double *grad_aux;  // [N][15] column-major 5x3 per cell, or null
     double *psi_aux;   // [N][5] or null
-    double *dt_cell;   // [N] per-cell CFL step (local time stepping), or null
 };
 
 // ORDER: 0 first order, 1 second order without limiter, 2 second order Barth
-template <int ORDER>
-__global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepA(DevMesh m, SweepAArgs a) {
-    __shared__ double sh[BLOCK / 64];
-    if (*a.stop) return;
-    const uint32_t N = m.N;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
-    const bool live = c0 < N;
-    const uint32_t c = live ? c0 : N - 1;  // dead lanes mirror the last cell, store nothing
+template <int ORDER, int B>
+__global__ __launch_bounds__(B) void k_sweepA(DevMesh m, SweepAArgs a) {
+    __shared__ double sh[B / 64];
+    __shared__ double lds[12 * B];
+    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
+    if (m.st[L.inst].stop) return;
+    const uint32_t c = L.c;
+    const bool live = L.live;
 
     // all loads first: own state, neighbour states (or ghost states), nodes
     const uint32_t code = m.fcode[c];
@@ -397,16 +429,10 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepA(DevMesh m, Swe
     }
     CellGeo g;
     load_cell(m, c, g);
+    stage_nodes<B>(lds, g);
     V3d S[4], r[4];
-#if NF_LDS_NODES
-    __shared__ double lds_nodes[12 * BLOCK];
-    stage_nodes(lds_nodes, g);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo_lds(m, g, lds_nodes, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
-#else
-#pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo(m, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
-#endif
+    for (int s = 0; s < 4; ++s) face_geo<B>(m, g, lds, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     __builtin_amdgcn_sched_barrier(0);
 
     if (ORDER == 0) {
@@ -462,66 +488,45 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepA(DevMesh m, Swe
         }
     }
     if (a.dt_part) {
-        // Delta S over interior faces (src/Solver.cpp:287-297) and the CFL
-        // step of the cell (src/Solver.cpp:254-266)
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (!(w[s] & 0x80000000u)) {
-                d0 += 0.5 * fabs(S[s].x);
-                d1 += 0.5 * fabs(S[s].y);
-                d2 += 0.5 * fabs(S[s].z);
-            }
-        }
-        const double cs = sound_speed_prim(Vc);
-        const double sx = (fabs(Vc[1]) + cs) * d0;
-        const double sy = (fabs(Vc[2]) + cs) * d1;
-        const double sz = (fabs(Vc[3]) + cs) * d2;
-        const double t = a.CFL * g.vol / ((sx + sy) + sz);
+        const double t = cell_dt(Vc, S, w, g.vol, a.CFL);
         const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;  // NaN/+inf never win std::min
         if (a.dt_cell && live) a.dt_cell[c] = t;
-        const double v = block_reduce<true>(dtc, sh);
-        if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
+        const double v = block_reduce<true, B>(dtc, sh);
+        if (threadIdx.x == 0) a.dt_part[L.lb] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
 struct SweepBArgs {
-    const double *FV;  // [tile][4][5][64]
-    const double *Gf;  // [nb][5] ghost of the reconstructed face state
-    const double *U;   // [tile][5][64] state entering the stage
-    const double *Uo;  // U_old, or null for zero
-    double *Uout;      // (may alias U)
-    double *Vout;      // prim(Uout)
-    const double *dt;  // device scalar
-    int kind;          // 1: U + dt/vol R, 2: RK3 stage 2, 3: RK3 stage 3
-    double *res_part;  // per-block sum of R_rho^2, or null
-    int *bad_flux;     // count of cells with non-finite R (if check)
-    int check;
-    int *bad_state;    // count of cells with rho<=0, p<=0 or NaN in Uout
-    const int *stop;
-    int xcd_remap;
-    double *R_aux;     // [N][5] or null: store R, skip the update
+    const double *FV;       // [tile][4][5][64]
+    const double *Gf;       // [nb][5] ghost of the reconstructed face state
+    const double *U;        // [tile][5][64] state entering the stage
+    const double *Uo;       // U_old (U^n)
+    double *Uout;           // (may alias U)
+    double *Vout;           // prim(Uout)
     const double *dt_cell;  // per-cell step (local time stepping), or null
+    int kind;               // 1: U + dt/vol R, 2: RK3 stage 2, 3: RK3 stage 3
+    double *res_part;       // per-block sum of R_rho^2, or null
+    int check;
+    int xcd_remap;
+    double *R_aux;          // [N][5] or null: store R, skip the update
 };
 
-__global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, SweepBArgs a) {
-    __shared__ double sh[BLOCK / 64];
-    if (*a.stop) return;
-    const uint32_t N = m.N;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
-    const bool live = c0 < N;
-    const uint32_t c = live ? c0 : N - 1;
+template <int B>
+__global__ __launch_bounds__(B) void k_sweepB(DevMesh m, SweepBArgs a) {
+    __shared__ double sh[B / 64];
+    __shared__ double lds[12 * B];
+    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
+    InstState &st = m.st[L.inst];
+    if (st.stop) return;
+    const uint32_t c = L.c;
+    const bool live = L.live;
 
-#ifndef NF_B_PRELOAD
-#define NF_B_PRELOAD 1
-#endif
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
     // face values of both sides of the four faces, all loads issued up front
     double Vo4[4][5], Vn4[4][5];
-#if NF_B_PRELOAD
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
@@ -534,45 +539,20 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
             Vn4[s][k] = src[k * stride];
         }
     }
-#endif
     CellGeo g;
     load_cell(m, c, g);
+    stage_nodes<B>(lds, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#if NF_LDS_NODES
-    __shared__ double lds_nodes[12 * BLOCK];
-    stage_nodes(lds_nodes, g);
-#endif
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t byte = (code >> (8 * s)) & 0xFFu;
-        const bool owner = byte & 0x80u;
         V3d S;
-#if NF_LDS_NODES
-        face_geo_lds(m, g, lds_nodes, byte, S, nullptr);
-#else
-        face_geo(m, g, byte, S, nullptr);
-#endif
-        double Vo[5], Vn[5];
-#if NF_B_PRELOAD
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            Vo[k] = Vo4[s][k];
-            Vn[k] = Vn4[s][k];
-        }
-#else
-        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
-        const bool bnd = w[s] & 0x80000000u;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            Vo[k] = a.FV[fix(c, s, k)];
-            Vn[k] = bnd ? a.Gf[5 * (size_t)idx + k] : a.FV[fix(idx, sn, k)];
-        }
-#endif
+        face_geo<B>(m, g, lds, byte, S, nullptr);
         // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
         double UO[5], UN[5], F[5];
-        prim_to_cons(Vo, UO);
-        prim_to_cons(Vn, UN);
-        const uint32_t mneg = owner ? 0u : 0x80000000u;
+        prim_to_cons(Vo4[s], UO);
+        prim_to_cons(Vn4[s], UN);
+        const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
         rusanov_side(UO, UN, S, mneg, F);
 #pragma unroll
         for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
@@ -582,7 +562,7 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < 5; ++k) ok = ok && (R[k] - R[k] == 0.0);
-        if (!ok) atomicAdd(a.bad_flux, 1);
+        if (!ok) atomicAdd(&st.bad_flux, 1);
     }
     if (a.R_aux) {
         if (live) {
@@ -590,7 +570,8 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
             for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
         }
     } else {
-        const double dt = a.dt_cell ? a.dt_cell[c] : *a.dt;
+        const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
+        const bool zero_old = st.uold_zero;
         double Un[5], Vnew[5];
         // Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:98-132)
         if (a.kind == 1) {
@@ -601,14 +582,14 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
             const double f = rdiv(1.0 / 4.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
                 Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
             }
         } else {
             const double f = rdiv(2.0 / 3.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
                 Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
             }
         }
@@ -619,13 +600,13 @@ __global__ __launch_bounds__(BLOCK, NF_SWEEP_WAVES) void k_sweepB(DevMesh m, Swe
                 a.Uout[tix(c, k)] = Un[k];
                 a.Vout[tix(c, k)] = Vnew[k];
             }
-            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(a.bad_state, 1);
+            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
         }
     }
     if (a.res_part) {
         const double r2 = live ? R[0] * R[0] : 0.0;
-        const double v = block_reduce<false>(r2, sh);
-        if (threadIdx.x == 0) a.res_part[blockIdx.x] = v;
+        const double v = block_reduce<false, B>(r2, sh);
+        if (threadIdx.x == 0) a.res_part[L.lb] = v;
     }
 }
 
@@ -640,7 +621,6 @@ __global__ __launch_bounds__(BLOCK) void k_prim(const double *U, double *V, uint
 #pragma unroll
     for (int k = 0; k < 5; ++k) V[tix(c, k)] = v[k];
 }
-
 #endif  // NF_DEVICE_HELPERS_ONLY
 
 }  // namespace nf

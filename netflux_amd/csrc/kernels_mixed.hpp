@@ -19,11 +19,11 @@ namespace mx {
 struct GhostArgsMx {
     const uint32_t *bface;
     const uint8_t *bbc;
+    const uint16_t *binst;
     uint32_t nb;
-    const float *src;  // V32 [tile][5][64] (cell) or FV32 (face)
-    float *out;        // [nb][5]
-    FarField<float> fs;
-    const int *stop;
+    const float *src;          // V32 [tile][5][64] (cell) or FV32 (face)
+    float *out;                // [nb][5]
+    const FarField<float> *fs; // [instances]
 };
 
 struct SweepAArgsMx {
@@ -31,11 +31,10 @@ struct SweepAArgsMx {
     const float *Gc;
     float *FV;
     double *dt_part;
+    double *dt_cell;
     double CFL;
-    const int *stop;
     int xcd_remap;
     double *grad_aux, *psi_aux;
-    double *dt_cell;
 };
 
 struct SweepBArgsMx {
@@ -45,16 +44,12 @@ struct SweepBArgsMx {
     const double *Uo;
     double *Uout;
     float *Vout;
-    const double *dt;
+    const double *dt_cell;
     int kind;
     double *res_part;
-    int *bad_flux;
     int check;
-    int *bad_state;
-    const int *stop;
     int xcd_remap;
     double *R_aux;
-    const double *dt_cell;
 };
 
 #ifdef NF_MIXED_KERNELS
@@ -68,8 +63,10 @@ __device__ __forceinline__ V3f to_f(V3d v) { return {(float)v.x, (float)v.y, (fl
 
 // S (FP32, from FP64 edge vectors of the face's own node order) and, if r is
 // given, the FP32 centroid-to-face vector
+template <int B>
 __device__ __forceinline__ void face_geo_mx(const double *lds, const CellGeo &g, uint32_t byte, V3f &S, V3f *r) {
-    const V3d a = lds_node(lds, byte & 3u), b = lds_node(lds, (byte >> 2) & 3u), c = lds_node(lds, (byte >> 4) & 3u);
+    const V3d a = lds_node<B>(lds, byte & 3u), b = lds_node<B>(lds, (byte >> 2) & 3u),
+              c = lds_node<B>(lds, (byte >> 4) & 3u);
     const V3f e1 = to_f(vsub(b, a)), e2 = to_f(vsub(c, a));
     S = {0.5f * (e1.y * e2.z - e1.z * e2.y), 0.5f * (e1.z * e2.x - e1.x * e2.z), 0.5f * (e1.x * e2.y - e1.y * e2.x)};
     if (byte & 0x40u) S = {-S.x, -S.y, -S.z};
@@ -125,36 +122,34 @@ __device__ __forceinline__ float barth_phi_f(float delta, float vmax, float vmin
 template <bool FACE>
 __global__ __launch_bounds__(BLOCK) void k_ghost_mx(DevMesh m, GhostArgsMx a) {
     __shared__ double lds[12 * BLOCK];
-    if (*a.stop) return;
     const uint32_t b0 = blockIdx.x * BLOCK + threadIdx.x;
     const bool live = b0 < a.nb;
     const uint32_t b = live ? b0 : a.nb - 1;
+    const uint32_t inst = a.binst[b];
     const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
     CellGeo g;
     load_cell(m, c, g);
-    stage_nodes(lds, g);
+    stage_nodes<BLOCK>(lds, g);
     V3f S;
-    face_geo_mx(lds, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    face_geo_mx<BLOCK>(lds, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
     float Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
-    ghost_value<float>((int)a.bbc[b], Vd, S, a.fs, Vg);
-    if (live) {
+    ghost_value<float>((int)a.bbc[b], Vd, S, a.fs[inst], Vg);
+    if (live && !m.st[inst].stop) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
     }
 }
 
-
-template <int ORDER>
-__global__ __launch_bounds__(BLOCK) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
-    __shared__ double sh[BLOCK / 64];
-    __shared__ double lds[12 * BLOCK];
-    if (*a.stop) return;
-    const uint32_t N = m.N;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
-    const bool live = c0 < N;
-    const uint32_t c = live ? c0 : N - 1;
+template <int ORDER, int B>
+__global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
+    __shared__ double sh[B / 64];
+    __shared__ double lds[12 * B];
+    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
+    if (m.st[L.inst].stop) return;
+    const uint32_t c = L.c;
+    const bool live = L.live;
 
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
@@ -175,10 +170,10 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) 
     }
     CellGeo g;
     load_cell(m, c, g);
-    stage_nodes(lds, g);
+    stage_nodes<B>(lds, g);
     V3f S[4], r[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo_mx(lds, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+    for (int s = 0; s < 4; ++s) face_geo_mx<B>(lds, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     const float ivol = (float)(1.0 / g.vol);
 
     if (ORDER == 0) {
@@ -224,37 +219,26 @@ __global__ __launch_bounds__(BLOCK) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) 
         }
     }
     if (a.dt_part) {
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (!(w[s] & 0x80000000u)) {
-                d0 += 0.5 * fabs((double)S[s].x);
-                d1 += 0.5 * fabs((double)S[s].y);
-                d2 += 0.5 * fabs((double)S[s].z);
-            }
-        }
-        const double V0 = Vc[0], V4 = Vc[4];
-        const double cs = sqrt(Consts<double>::GAMMA * V4 / V0);
-        const double sx = (fabs((double)Vc[1]) + cs) * d0;
-        const double sy = (fabs((double)Vc[2]) + cs) * d1;
-        const double sz = (fabs((double)Vc[3]) + cs) * d2;
-        const double t = a.CFL * g.vol / ((sx + sy) + sz);
+        const V3d Sd[4] = {{S[0].x, S[0].y, S[0].z}, {S[1].x, S[1].y, S[1].z}, {S[2].x, S[2].y, S[2].z},
+                           {S[3].x, S[3].y, S[3].z}};
+        const double Vd[5] = {Vc[0], Vc[1], Vc[2], Vc[3], Vc[4]};
+        const double t = cell_dt(Vd, Sd, w, g.vol, a.CFL);
         const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;
         if (a.dt_cell && live) a.dt_cell[c] = t;
-        const double v = block_reduce<true>(dtc, sh);
-        if (threadIdx.x == 0) a.dt_part[blockIdx.x] = v;
+        const double v = block_reduce<true, B>(dtc, sh);
+        if (threadIdx.x == 0) a.dt_part[L.lb] = v;
     }
 }
 
-
-__global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
-    __shared__ double sh[BLOCK / 64];
-    __shared__ double lds[12 * BLOCK];
-    if (*a.stop) return;
-    const uint32_t N = m.N;
-    const uint32_t c0 = logical_block(blockIdx.x, gridDim.x, a.xcd_remap) * BLOCK + threadIdx.x;
-    const bool live = c0 < N;
-    const uint32_t c = live ? c0 : N - 1;
+template <int B>
+__global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
+    __shared__ double sh[B / 64];
+    __shared__ double lds[12 * B];
+    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
+    InstState &st = m.st[L.inst];
+    if (st.stop) return;
+    const uint32_t c = L.c;
+    const bool live = L.live;
 
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
@@ -274,13 +258,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
     }
     CellGeo g;
     load_cell(m, c, g);
-    stage_nodes(lds, g);
+    stage_nodes<B>(lds, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t byte = (code >> (8 * s)) & 0xFFu;
         V3f S;
-        face_geo_mx(lds, g, byte, S, nullptr);
+        face_geo_mx<B>(lds, g, byte, S, nullptr);
         float UO[5], UN[5], F[5];
         prim_to_cons_f(Vo4[s], UO);
         prim_to_cons_f(Vn4[s], UN);
@@ -293,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < 5; ++k) ok = ok && (R[k] - R[k] == 0.0);
-        if (!ok) atomicAdd(a.bad_flux, 1);
+        if (!ok) atomicAdd(&st.bad_flux, 1);
     }
     if (a.R_aux) {
         if (live) {
@@ -301,7 +285,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
             for (int k = 0; k < 5; ++k) a.R_aux[5 * (size_t)c + k] = R[k];
         }
     } else {
-        const double dt = a.dt_cell ? a.dt_cell[c] : *a.dt;
+        const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
+        const bool zero_old = st.uold_zero;
         double Un[5], Vnew[5];
         if (a.kind == 1) {
             const double f = dt / g.vol;
@@ -311,14 +296,14 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
             const double f = 1.0 / 4.0 * dt / g.vol;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
                 Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
             }
         } else {
             const double f = 2.0 / 3.0 * dt / g.vol;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = a.Uo ? a.Uo[tix(c, k)] : 0.0;
+                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
                 Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
             }
         }
@@ -329,13 +314,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) 
                 a.Uout[tix(c, k)] = Un[k];
                 a.Vout[tix(c, k)] = (float)Vnew[k];
             }
-            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(a.bad_state, 1);
+            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
         }
     }
     if (a.res_part) {
         const double r2 = live ? R[0] * R[0] : 0.0;
-        const double v = block_reduce<false>(r2, sh);
-        if (threadIdx.x == 0) a.res_part[blockIdx.x] = v;
+        const double v = block_reduce<false, B>(r2, sh);
+        if (threadIdx.x == 0) a.res_part[L.lb] = v;
     }
 }
 
@@ -354,8 +339,8 @@ __global__ __launch_bounds__(BLOCK) void k_prim_mx(const double *U, float *V, ui
 
 // host-side launchers (compiled in solver_mixed.hip)
 hipError_t launch_ghost_mx(bool face, dim3 g, hipStream_t st, const DevMesh &m, const GhostArgsMx &a);
-hipError_t launch_sweepA_mx(int order, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a);
-hipError_t launch_sweepB_mx(dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a);
+hipError_t launch_sweepA_mx(int order, int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a);
+hipError_t launch_sweepB_mx(int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a);
 hipError_t launch_prim_mx(dim3 g, hipStream_t st, const double *U, float *V, uint32_t N);
 
 }  // namespace mx

@@ -1,13 +1,14 @@
 // Solver context and C ABI (include/netflux.h).
 //
 // The context replaces EulerSolver + EulerSolverData (include/Solver.hpp:52-79,
-// include/SolverData.hpp:88-117): it owns the device fields and runs the
-// Driver loop order (calc_timestep, then step; src/Driver.cpp:38-47) with dt
-// kept on the device.
+// include/SolverData.hpp:88-117): it owns the device fields of one or more
+// meshes (instances) and runs the Driver loop order (calc_timestep, then
+// step; src/Driver.cpp:38-47) with the time steps kept on the device.  All
+// instances of a context advance together: one launch per sweep covers every
+// instance, each with its own dt, run counters and stop flag.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -24,80 +25,109 @@ using namespace nf;
 
 namespace {
 
-// First level of the per-block partial reductions: block b reduces the
-// contiguous chunk [b*RCHUNK, (b+1)*RCHUNK) in a fixed order.
-constexpr int RCHUNK = 2048;
+constexpr uint32_t RCHUNK = 2048;  // blocks per first-level reduction chunk
+
+// First level of the per-block partial reductions: chunk q reduces blocks
+// [first[q], first[q] + n[q]) of one instance in a fixed order.
 template <bool MIN>
-__global__ __launch_bounds__(BLOCK) void k_reduce(const double *in, int n, double *out, const int *stop) {
+__global__ __launch_bounds__(BLOCK) void k_reduce(const double *in, const uint32_t *first, const uint32_t *n,
+                                                  double *out) {
     __shared__ double sh[BLOCK / 64];
-    if (*stop) return;
-    const int base = blockIdx.x * RCHUNK + threadIdx.x * (RCHUNK / BLOCK);
+    const uint32_t q = blockIdx.x, b0 = first[q], nb = n[q];
     double v = MIN ? DBL_MAX : 0.0;
-    for (int q = 0; q < RCHUNK / BLOCK; ++q) {
-        const int i = base + q;
-        if (i < n) v = MIN ? ((in[i] < v) ? in[i] : v) : v + in[i];
+    for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) {
+        const double x = in[b0 + i];
+        v = MIN ? ((x < v) ? x : v) : v + x;
     }
-    v = block_reduce<MIN>(v, sh);
-    if (threadIdx.x == 0) out[blockIdx.x] = v;
+    v = block_reduce<MIN, BLOCK>(v, sh);
+    if (threadIdx.x == 0) out[q] = v;
 }
 
-__global__ void k_finalize_dt(const double *part, int nb, double *dt, double *time_acc,
-                              int *nstep, int *err, int *stop, int advance) {
+// Second level, one block per instance: dt = min over the instance's chunks.
+// advance = 1 at the end of a step: time += dt, nstep += 1, the first-step
+// flag clears, a converged instance stops, and the new dt (of the primvars
+// the step's last sweep A read) is installed for the next step.
+__global__ __launch_bounds__(BLOCK) void k_finalize_dt(const double *red, const uint32_t *c0, const uint32_t *nc,
+                                                       InstState *st, int advance) {
     __shared__ double sh[BLOCK / 64];
-    if (advance && *stop) return;
+    const uint32_t i = blockIdx.x;
+    if (st[i].stop) return;
     double v = DBL_MAX;
-    for (int b = threadIdx.x; b < nb; b += BLOCK) v = (part[b] < v) ? part[b] : v;
-    v = block_reduce<true>(v, sh);
+    for (uint32_t q = threadIdx.x; q < nc[i]; q += BLOCK) {
+        const double x = red[c0[i] + q];
+        v = (x < v) ? x : v;
+    }
+    v = block_reduce<true, BLOCK>(v, sh);
     if (threadIdx.x == 0) {
         if (advance) {
-            *time_acc += *dt;
-            *nstep += 1;
+            st[i].time += st[i].dt;
+            st[i].nstep += 1;
+            st[i].uold_zero = 0;
+            if (st[i].converged) st[i].stop = 1;
         }
         // num_is_valid_and_pos (include/Includes.hpp:131-137)
         const bool ok = (v - v == 0.0) && v > 0.0;
-        *dt = v;
+        st[i].dt = v;
         if (!ok) {
-            *err = NF_E_INVALID_DT;
-            *stop = 1;
+            st[i].err = NF_E_INVALID_DT;
+            st[i].stop = 1;
         }
     }
 }
 
-// r_k = sqrt(sum of per-block partials), summed in a fixed order.
-__global__ void k_finalize_res(const double *part, int nb, double *hist, int hist_cap,
-                               const int *nstep, int *stop, double drop, int *converged,
-                               int check_now) {
+// r_k = sqrt(sum of the instance's partials), summed in a fixed order, into
+// the instance's residual history; sets `converged` once r_k / r_0 <= drop.
+__global__ __launch_bounds__(BLOCK) void k_finalize_res(const double *red, const uint32_t *c0, const uint32_t *nc,
+                                                        InstState *st, double *hist, int cap, double drop) {
     __shared__ double sh[BLOCK / 64];
-    if (*stop) return;
+    const uint32_t i = blockIdx.x;
+    if (st[i].stop) return;
     double v = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BLOCK) v += part[b];
-    v = block_reduce<false>(v, sh);
+    for (uint32_t q = threadIdx.x; q < nc[i]; q += BLOCK) v += red[c0[i] + q];
+    v = block_reduce<false, BLOCK>(v, sh);
     if (threadIdx.x == 0) {
-        const int k = *nstep;
+        const int k = st[i].nstep;
         const double r = sqrt(v);
-        if (k < hist_cap) hist[k] = r;
-        const double r1 = hist[0];
-        if (check_now && drop > 0.0 && k < hist_cap && r1 > 0.0 && r / r1 <= drop) *converged = 1;
+        double *h = hist + (size_t)i * cap;
+        if (k < cap) h[k] = r;
+        if (drop > 0.0 && k < cap && h[0] > 0.0 && r / h[0] <= drop) st[i].converged = 1;
     }
 }
 
-__global__ void k_stop_if_converged(const int *converged, int *stop) {
-    if (*converged) *stop = 1;
+// op 0: clear the run flags of every instance (err, stop, converged, error
+// counters); op 1: set dt of instance `which`
+__global__ void k_flags(InstState *st, int n, int op, int which, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) {
+        st[i].err = 0;
+        st[i].stop = 0;
+        st[i].converged = 0;
+        st[i].bad_flux = 0;
+        st[i].bad_state = 0;
+    } else if (i == which) {
+        st[i].dt = dt;
+    }
 }
 
-// flags [first, first+n) = 0 and, if set_dt, scal[0] = dt
-__global__ void k_set_flags(int *iscal, int first, int n, double *scal, int set_dt, double dt,
-                            int clear_scal) {
-    const int t = threadIdx.x;
-    if (t < n) iscal[first + t] = 0;
-    if (t == 0 && set_dt) scal[0] = dt;
-    if (t < 2 && clear_scal) scal[t] = 0.0;
+// fresh run state of instance i (nstep = 0, time = 0)
+__global__ void k_reset_inst(InstState *st, int i, int uold_zero) {
+    if (threadIdx.x == 0) st[i] = InstState{0.0, 0.0, 0, 0, 0, 0, 0, 0, uold_zero, 0};
 }
 
+struct Inst {
+    uint32_t cell0 = 0, N = 0, Npad = 0;  // device cells [cell0, cell0 + N), block [cell0, cell0 + Npad)
+    nf_fv_sizes sizes{};
+    std::vector<uint32_t> perm;  // local device cell -> reference cell
+    double Vinf[5] = {0, 0, 0, 0, 0};
+    bool dt_valid = false;
+};
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
+// reduction chunks of whole instances for one sweep block size
+struct RedTable {
+    int nchunk = 0;
+    uint32_t *first = nullptr, *n = nullptr;          // [nchunk]
+    uint32_t *inst_c0 = nullptr, *inst_nc = nullptr;  // [instances] chunk range
 };
 
 }  // namespace
@@ -107,41 +137,38 @@ struct nf_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    uint32_t N = 0, n_nodes = 0;
-    nf_fv_sizes sizes{};
-    std::vector<uint32_t> perm, iperm;  // device <-> reference cell
-    FarField<double> fs{};
-    int nb = 0;          // blocks of a sweep
-    uint32_t npad = 0;   // N rounded up to whole 64-cell tiles
-    uint32_t nbf = 0;    // boundary faces
-    uint32_t *bface = nullptr;
-    uint8_t *bbc = nullptr;
-    double *Gc = nullptr, *Gf = nullptr;  // ghost states [nbf][5]
-    bool mixed = false;                   // NF_PREC_MIXED: FP32 copies below
-    float *V32 = nullptr, *FV32 = nullptr, *Gc32 = nullptr, *Gf32 = nullptr;
-    double *dt_cell = nullptr;            // local time stepping: per-cell step
-    FarField<float> fs32{};
-    int xcd_remap = 1;
-    // device arrays
+    int n_inst = 0;
+    std::vector<Inst> inst;
+    uint32_t Ncell = 0;  // device cells including padding (multiple of 256)
+    int block = 256, xcd_remap = 1;  // sweep launch configuration
+    bool mixed = false;
     double *nodes = nullptr;
     uint32_t *cnodes = nullptr, *nbr = nullptr, *fcode = nullptr;
+    uint16_t *blk_inst = nullptr;
+    uint32_t *inst_end = nullptr;
+    InstState *st = nullptr;
     double *Ua = nullptr, *Ub = nullptr, *V = nullptr, *FV = nullptr;
+    float *V32 = nullptr, *FV32 = nullptr;
+    uint32_t nbf = 0;
+    uint32_t *bface = nullptr;
+    uint8_t *bbc = nullptr;
+    uint16_t *binst = nullptr;
+    double *Gc = nullptr, *Gf = nullptr;
+    float *Gc32 = nullptr, *Gf32 = nullptr;
+    FarField<double> *fs = nullptr;
+    FarField<float> *fs32 = nullptr;
+    double *dt_cell = nullptr;
     double *grad_aux = nullptr, *psi_aux = nullptr, *R_aux = nullptr;
-    double *dt_part = nullptr, *res_part = nullptr, *red2 = nullptr;
-    int nb2 = 0;  // second-level partials
-    double *scal = nullptr;  // [0] dt, [1] time
-    int *iscal = nullptr;    // [0] nstep [1] err [2] stop [3] converged [4] bad_flux [5] bad_state
-    double *hist = nullptr;
+    double *dt_part = nullptr, *res_part = nullptr;  // per sweep block
+    RedTable red256, red128;
+    double *red_dt = nullptr, *red_res = nullptr;  // per chunk
+    double *hist = nullptr;                         // [instances][hist_cap]
     int hist_cap = 0;
-    bool uold_zero = true;   // U_old is all zeros (before the first step)
-    bool dt_valid = false;   // scal[0] holds the dt of the primvars as last set
     bool aux_valid = false;
     std::vector<void *> owned;
-    // graph cache
     hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
+    int graph_steps = 0, graph_block = 0, graph_xcd = -1;
     double graph_drop = -1.0;
-    // timing
     bool timing = false;
     std::map<std::string, std::pair<double, int>> ktime;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -154,10 +181,10 @@ int fail(nf_ctx *c, int code, const std::string &msg) {
     return code;
 }
 
-#define HIPC(expr)                                                                     \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
+#define HIPC(expr)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
             return fail(ctx, NF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
@@ -171,27 +198,13 @@ int dalloc(nf_ctx *ctx, T **p, size_t n) {
     return NF_OK;
 }
 
-DevMesh devmesh(const nf_ctx *c) { return DevMesh{c->N, 3.0, 6.0, c->nodes, c->cnodes, c->nbr, c->fcode}; }
-
-// kernel launch with optional per-kernel event timing
-template <typename K, typename... Args>
-int launch(nf_ctx *ctx, const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ctx->timing) {
-        HIPC(hipEventCreate(&e0));
-        HIPC(hipEventCreate(&e1));
-        HIPC(hipEventRecord(e0, ctx->stream));
-    }
-    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
-    HIPC(hipGetLastError());
-    if (ctx->timing) {
-        HIPC(hipEventRecord(e1, ctx->stream));
-        ctx->pending.push_back({name, {e0, e1}});
-    }
-    return NF_OK;
+DevMesh devmesh(const nf_ctx *c) {
+    return DevMesh{3.0, 6.0, c->nodes, c->cnodes, c->nbr, c->fcode, c->blk_inst, c->inst_end, c->st};
 }
 
-// same as launch() for a launcher callable returning hipError_t
+const RedTable &red(const nf_ctx *c) { return c->block == 128 ? c->red128 : c->red256; }
+
+// kernel launch with optional per-kernel event timing
 template <typename F>
 int launch_fn(nf_ctx *ctx, const char *name, F fn) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -207,6 +220,11 @@ int launch_fn(nf_ctx *ctx, const char *name, F fn) {
     }
     return NF_OK;
 }
+#define LAUNCH(name, kernel, grid, blk, ...)                                          \
+    launch_fn(ctx, name, [&] {                                                        \
+        hipLaunchKernelGGL(kernel, grid, blk, 0, ctx->stream, __VA_ARGS__);           \
+        return hipGetLastError();                                                     \
+    })
 
 int flush_timing(nf_ctx *ctx) {
     HIPC(hipStreamSynchronize(ctx->stream));
@@ -223,244 +241,606 @@ int flush_timing(nf_ctx *ctx) {
     return NF_OK;
 }
 
-SweepAArgs argsA(nf_ctx *ctx, bool dt, bool aux, bool dt_cell) {
-    SweepAArgs a{};
-    a.V = ctx->V;
-    a.Gc = ctx->Gc;
-    a.FV = ctx->FV;
-    a.dt_part = dt ? ctx->dt_part : nullptr;
-    a.CFL = ctx->prm.CFL;
-    a.stop = ctx->iscal + 2;
-    a.xcd_remap = ctx->xcd_remap;
-    a.grad_aux = aux ? ctx->grad_aux : nullptr;
-    a.psi_aux = aux ? ctx->psi_aux : nullptr;
-    a.dt_cell = dt_cell ? ctx->dt_cell : nullptr;
-    return a;
-}
-
 int run_ghost(nf_ctx *ctx, bool face) {
     if (ctx->nbf == 0) return NF_OK;
-    dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
+    const dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
+    const DevMesh m = devmesh(ctx);
     if (ctx->mixed) {
-        mx::GhostArgsMx a{ctx->bface, ctx->bbc, ctx->nbf, face ? ctx->FV32 : ctx->V32, face ? ctx->Gf32 : ctx->Gc32,
-                          ctx->fs32, ctx->iscal + 2};
+        const mx::GhostArgsMx a{ctx->bface, ctx->bbc, ctx->binst, ctx->nbf, face ? ctx->FV32 : ctx->V32,
+                                face ? ctx->Gf32 : ctx->Gc32, ctx->fs32};
         return launch_fn(ctx, face ? "ghost_face_mx" : "ghost_cell_mx",
-                         [&] { return mx::launch_ghost_mx(face, g, ctx->stream, devmesh(ctx), a); });
+                         [&] { return mx::launch_ghost_mx(face, g, ctx->stream, m, a); });
     }
-    GhostArgs a{ctx->bface, ctx->bbc, ctx->nbf, face ? ctx->FV : ctx->V, face ? ctx->Gf : ctx->Gc, ctx->fs,
-                ctx->iscal + 2};
-    if (face) return launch(ctx, "ghost_face", k_ghost<true>, g, b, devmesh(ctx), a);
-    return launch(ctx, "ghost_cell", k_ghost<false>, g, b, devmesh(ctx), a);
+    const GhostArgs a{ctx->bface, ctx->bbc, ctx->binst, ctx->nbf, face ? ctx->FV : ctx->V,
+                      face ? ctx->Gf : ctx->Gc, ctx->fs};
+    if (face) return LAUNCH("ghost_face", k_ghost<true>, g, b, m, a);
+    return LAUNCH("ghost_cell", k_ghost<false>, g, b, m, a);
 }
 
-int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dt_cell = false) {
+int order_of(const nf_ctx *ctx) {
+    if (ctx->prm.spatial_order == NF_FIRST_ORDER) return 0;
+    return ctx->prm.limiter == NF_LIMITER_NONE ? 1 : 2;
+}
+
+template <int B>
+hipError_t launch_sweepA(hipStream_t s, uint32_t ncell, int order, const DevMesh &m, const SweepAArgs &a) {
+    const dim3 g(ncell / B), b(B);
+    if (order == 0) hipLaunchKernelGGL((k_sweepA<0, B>), g, b, 0, s, m, a);
+    else if (order == 1) hipLaunchKernelGGL((k_sweepA<1, B>), g, b, 0, s, m, a);
+    else hipLaunchKernelGGL((k_sweepA<2, B>), g, b, 0, s, m, a);
+    return hipGetLastError();
+}
+
+// sweep A of the current primvars.  dt: per-block CFL minima into dt_part;
+// dtc: per-cell CFL steps into dt_cell (local time stepping); aux: keep the
+// gradient and limiter fields.
+int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dtc = false) {
     int rc;
     if ((rc = run_ghost(ctx, false))) return rc;
     const DevMesh m = devmesh(ctx);
-    dim3 g(ctx->nb), b(BLOCK);
+    const int order = order_of(ctx);
     if (ctx->mixed) {
-        mx::SweepAArgsMx x{ctx->V32, ctx->Gc32, ctx->FV32, dt ? ctx->dt_part : nullptr, ctx->prm.CFL,
-                           ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->grad_aux : nullptr,
-                           aux ? ctx->psi_aux : nullptr, dt_cell ? ctx->dt_cell : nullptr};
-        const int order = ctx->prm.spatial_order == NF_FIRST_ORDER ? 0 : (ctx->prm.limiter == NF_LIMITER_NONE ? 1 : 2);
-        return launch_fn(ctx, order == 2 ? "sweepA_barth_mx" : "sweepA_mx",
-                         [&] { return mx::launch_sweepA_mx(order, g, ctx->stream, m, x); });
+        const mx::SweepAArgsMx x{ctx->V32, ctx->Gc32, ctx->FV32, dt ? ctx->dt_part : nullptr,
+                                 dtc ? ctx->dt_cell : nullptr, ctx->prm.CFL, ctx->xcd_remap,
+                                 aux ? ctx->grad_aux : nullptr, aux ? ctx->psi_aux : nullptr};
+        return launch_fn(ctx, order == 2 ? "sweepA_barth_mx" : "sweepA_mx", [&] {
+            return mx::launch_sweepA_mx(order, ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
+        });
     }
-    SweepAArgs a = argsA(ctx, dt, aux, dt_cell);
-    if (ctx->prm.spatial_order == NF_FIRST_ORDER)
-        return launch(ctx, "sweepA_first", k_sweepA<0>, g, b, m, a);
-    if (ctx->prm.limiter == NF_LIMITER_NONE)
-        return launch(ctx, "sweepA_nolim", k_sweepA<1>, g, b, m, a);
-    return launch(ctx, "sweepA_barth", k_sweepA<2>, g, b, m, a);
+    const SweepAArgs a{ctx->V, ctx->Gc, ctx->FV, dt ? ctx->dt_part : nullptr, dtc ? ctx->dt_cell : nullptr,
+                       ctx->prm.CFL, ctx->xcd_remap, aux ? ctx->grad_aux : nullptr, aux ? ctx->psi_aux : nullptr};
+    const char *name = order == 2 ? "sweepA_barth" : (order == 1 ? "sweepA_nolim" : "sweepA_first");
+    return launch_fn(ctx, name, [&] {
+        return ctx->block == 128 ? launch_sweepA<128>(ctx->stream, ctx->Ncell, order, m, a)
+                                 : launch_sweepA<256>(ctx->stream, ctx->Ncell, order, m, a);
+    });
 }
 
-int run_sweepB(nf_ctx *ctx, int kind, const double *U, const double *Uo, double *Uout, bool res,
-               bool aux) {
+// sweep B: stage `kind` of the update from U into Uout, prim(Uout) into V.
+// res: per-block sums of R_rho^2; aux: keep R.
+int run_sweepB(nf_ctx *ctx, int kind, const double *U, double *Uout, bool res, bool aux) {
     int rc;
     if ((rc = run_ghost(ctx, true))) return rc;
+    const DevMesh m = devmesh(ctx);
+    const double *dtc = ctx->prm.local_time_step ? ctx->dt_cell : nullptr;
     if (ctx->mixed) {
-        mx::SweepBArgsMx x{ctx->FV32, ctx->Gf32, U, Uo, Uout, ctx->V32, ctx->scal, kind,
-                           res ? ctx->res_part : nullptr, ctx->iscal + 4, ctx->prm.check_physical_validity,
-                           ctx->iscal + 5, ctx->iscal + 2, ctx->xcd_remap, aux ? ctx->R_aux : nullptr,
-                           ctx->prm.local_time_step ? ctx->dt_cell : nullptr};
-        return launch_fn(ctx, "sweepB_mx",
-                         [&] { return mx::launch_sweepB_mx(dim3(ctx->nb), ctx->stream, devmesh(ctx), x); });
+        const mx::SweepBArgsMx x{ctx->FV32, ctx->Gf32, U, ctx->Ua, Uout, ctx->V32, dtc, kind,
+                                 res ? ctx->res_part : nullptr, ctx->prm.check_physical_validity,
+                                 ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+        return launch_fn(ctx, "sweepB_mx", [&] {
+            return mx::launch_sweepB_mx(ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
+        });
     }
-    SweepBArgs a{};
-    a.FV = ctx->FV;
-    a.Gf = ctx->Gf;
-    a.U = U;
-    a.Uo = Uo;
-    a.Uout = Uout;
-    a.Vout = ctx->V;
-    a.dt = ctx->scal;
-    a.kind = kind;
-    a.res_part = res ? ctx->res_part : nullptr;
-    a.bad_flux = ctx->iscal + 4;
-    a.check = ctx->prm.check_physical_validity;
-    a.bad_state = ctx->iscal + 5;
-    a.stop = ctx->iscal + 2;
-    a.xcd_remap = ctx->xcd_remap;
-    a.R_aux = aux ? ctx->R_aux : nullptr;
-    a.dt_cell = ctx->prm.local_time_step ? ctx->dt_cell : nullptr;
-    return launch(ctx, "sweepB", k_sweepB, dim3(ctx->nb), dim3(BLOCK), devmesh(ctx), a);
+    const SweepBArgs a{ctx->FV, ctx->Gf, U, ctx->Ua, Uout, ctx->V, dtc, kind, res ? ctx->res_part : nullptr,
+                       ctx->prm.check_physical_validity, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+    return launch_fn(ctx, "sweepB", [&] {
+        if (ctx->block == 128)
+            hipLaunchKernelGGL(k_sweepB<128>, dim3(ctx->Ncell / 128), dim3(128), 0, ctx->stream, m, a);
+        else hipLaunchKernelGGL(k_sweepB<256>, dim3(ctx->Ncell / 256), dim3(256), 0, ctx->stream, m, a);
+        return hipGetLastError();
+    });
 }
 
-// One solver step on the device (Solver::step, src/Solver.cpp:34-52).  The
+int reduce_dt(nf_ctx *ctx, int advance) {
+    int rc;
+    const RedTable &r = red(ctx);
+    if ((rc = LAUNCH("reduce_dt", k_reduce<true>, dim3(r.nchunk), dim3(BLOCK), (const double *)ctx->dt_part,
+                     (const uint32_t *)r.first, (const uint32_t *)r.n, ctx->red_dt)))
+        return rc;
+    return LAUNCH("finalize_dt", k_finalize_dt, dim3(ctx->n_inst), dim3(BLOCK), (const double *)ctx->red_dt,
+                  (const uint32_t *)r.inst_c0, (const uint32_t *)r.inst_nc, ctx->st, advance);
+}
+
+int reduce_res(nf_ctx *ctx, double drop) {
+    int rc;
+    const RedTable &r = red(ctx);
+    if ((rc = LAUNCH("reduce_res", k_reduce<false>, dim3(r.nchunk), dim3(BLOCK), (const double *)ctx->res_part,
+                     (const uint32_t *)r.first, (const uint32_t *)r.n, ctx->red_res)))
+        return rc;
+    return LAUNCH("finalize_res", k_finalize_res, dim3(ctx->n_inst), dim3(BLOCK), (const double *)ctx->red_res,
+                  (const uint32_t *)r.inst_c0, (const uint32_t *)r.inst_nc, ctx->st, ctx->hist, ctx->hist_cap,
+                  drop);
+}
+
+// One step of every live instance (Solver::step, src/Solver.cpp:34-52).  The
 // last sweep A of the step evaluates the CFL dt of the primvars it reads --
-// prim(U2) for TVD-RK3, prim(U^n) for explicit Euler -- which is exactly the
-// state calc_timestep sees next (src/Solver.cpp:248).  k_finalize_dt then
-// advances time and installs that dt for the following step.
-int enqueue_step(nf_ctx *ctx, bool zero_old, double drop, bool check_now) {
+// prim(U2) for TVD-RK3, prim(U^n) for explicit Euler -- which is the state
+// the reference's next calc_timestep sees (src/Solver.cpp:248);
+// k_finalize_dt installs that dt for the following step.  U^n (= U_old after
+// the first step) stays in Ua; stages write Ub, the last stage writes Ua.
+int enqueue_step(nf_ctx *ctx, double drop) {
     int rc;
     const bool rk3 = ctx->prm.time_scheme == NF_TVD_RK3;
     const bool lts = ctx->prm.local_time_step;
-    int *is = ctx->iscal;
-    // local time stepping: per-cell steps from the state entering the step
     if ((rc = run_sweepA(ctx, !rk3 || lts, false, lts))) return rc;
-    if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, rk3 ? ctx->Ub : ctx->Ua, true, false))) return rc;
-    if ((rc = launch(ctx, "reduce_res", k_reduce<false>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->res_part,
-                     ctx->nb, ctx->red2 + ctx->nb2, (const int *)(is + 2))))
-        return rc;
-    if ((rc = launch(ctx, "finalize_res", k_finalize_res, dim3(1), dim3(BLOCK), (const double *)(ctx->red2 + ctx->nb2),
-                     ctx->nb2, ctx->hist, ctx->hist_cap, (const int *)is, is + 2, drop, is + 3,
-                     (int)check_now)))
-        return rc;
+    if ((rc = run_sweepB(ctx, 1, ctx->Ua, rk3 ? ctx->Ub : ctx->Ua, true, false))) return rc;
+    if ((rc = reduce_res(ctx, drop))) return rc;
     if (rk3) {
-        const double *Uo = zero_old ? nullptr : ctx->Ua;
         if ((rc = run_sweepA(ctx, false, false))) return rc;
-        if ((rc = run_sweepB(ctx, 2, ctx->Ub, Uo, ctx->Ub, false, false))) return rc;
+        if ((rc = run_sweepB(ctx, 2, ctx->Ub, ctx->Ub, false, false))) return rc;
         if ((rc = run_sweepA(ctx, !lts, false))) return rc;
-        if ((rc = run_sweepB(ctx, 3, ctx->Ub, Uo, ctx->Ua, false, false))) return rc;
+        if ((rc = run_sweepB(ctx, 3, ctx->Ub, ctx->Ua, false, false))) return rc;
     }
-    if ((rc = launch(ctx, "reduce_dt", k_reduce<true>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->dt_part,
-                     ctx->nb, ctx->red2, (const int *)(is + 2))))
-        return rc;
-    if ((rc = launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->red2,
-                     ctx->nb2, ctx->scal, ctx->scal + 1, is, is + 1, is + 2, 1)))
-        return rc;
-    if (drop > 0.0)
-        if ((rc = launch(ctx, "stop_if_converged", k_stop_if_converged, dim3(1), dim3(1),
-                         (const int *)(is + 3), is + 2)))
-            return rc;
-    return NF_OK;
+    return reduce_dt(ctx, 1);
 }
 
-// dt of the current primvars with a standalone sweep A (no face values used)
+// dt of the current primvars with a standalone sweep A
 int enqueue_timestep(nf_ctx *ctx) {
     int rc;
     if ((rc = run_sweepA(ctx, true, false))) return rc;
-    if ((rc = launch(ctx, "reduce_dt", k_reduce<true>, dim3(ctx->nb2), dim3(BLOCK), (const double *)ctx->dt_part,
-                     ctx->nb, ctx->red2, (const int *)(ctx->iscal + 2))))
-        return rc;
-    return launch(ctx, "finalize_dt", k_finalize_dt, dim3(1), dim3(BLOCK), (const double *)ctx->red2,
-                  ctx->nb2, ctx->scal, ctx->scal + 1, ctx->iscal, ctx->iscal + 1, ctx->iscal + 2, 0);
-}
-
-int check_errors(nf_ctx *ctx) {
-    int is[6];
-    HIPC(hipMemcpyAsync(is, ctx->iscal, sizeof is, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
-    if (is[1] == NF_E_INVALID_DT) {
-        double dt;
-        HIPC(hipMemcpy(&dt, ctx->scal, sizeof dt, hipMemcpyDeviceToHost));
-        char buf[96];
-        snprintf(buf, sizeof buf, "Invalid dt calculated (dt = %f)", dt);
-        return fail(ctx, NF_E_INVALID_DT, buf);
-    }
-    if (is[4] > 0)
-        return fail(ctx, NF_E_NONPHYSICAL,
-                    std::to_string(is[4]) +
-                        " cells with unphysical values detected in the flux_balance field of the Euler solver");
-    return NF_OK;
+    return reduce_dt(ctx, 0);
 }
 
 int clear_flags(nf_ctx *ctx) {
-    hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->iscal, 1, 5, ctx->scal, 0, 0.0, 0);
+    hipLaunchKernelGGL(k_flags, dim3((ctx->n_inst + 63) / 64), dim3(64), 0, ctx->stream, ctx->st, ctx->n_inst, 0,
+                       0, 0.0);
     HIPC(hipGetLastError());
+    return NF_OK;
+}
+
+int read_states(nf_ctx *ctx, std::vector<InstState> &h) {
+    h.resize(ctx->n_inst);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(h.data(), ctx->st, h.size() * sizeof(InstState), hipMemcpyDeviceToHost));
+    return NF_OK;
+}
+
+// the reference's messages (src/Solver.cpp:267-268, src/SolverData.cpp:73-78);
+// for a batch, the first failing instance is named
+int check_errors(nf_ctx *ctx) {
+    std::vector<InstState> h;
+    int rc;
+    if ((rc = read_states(ctx, h))) return rc;
+    for (int i = 0; i < ctx->n_inst; ++i) {
+        const std::string who = ctx->n_inst > 1 ? "instance " + std::to_string(i) + ": " : "";
+        if (h[i].err == NF_E_INVALID_DT) {
+            char buf[96];
+            snprintf(buf, sizeof buf, "Invalid dt calculated (dt = %f)", h[i].dt);
+            return fail(ctx, NF_E_INVALID_DT, who + buf);
+        }
+        if (h[i].bad_flux > 0)
+            return fail(ctx, NF_E_NONPHYSICAL,
+                        who + std::to_string(h[i].bad_flux) +
+                            " cells with unphysical values detected in the flux_balance field of the Euler solver");
+    }
     return NF_OK;
 }
 
 int ensure_hist(nf_ctx *ctx, int need) {
     if (need <= ctx->hist_cap) return NF_OK;
-    int cap = std::max(need, 2 * ctx->hist_cap + 1024);
+    const int cap = std::max(need, 2 * ctx->hist_cap + 1024);
+    const size_t bytes = (size_t)cap * ctx->n_inst * sizeof(double);
     double *h = nullptr;
-    HIPC(hipMalloc(&h, (size_t)cap * sizeof(double)));
-    HIPC(hipMemsetAsync(h, 0, (size_t)cap * sizeof(double), ctx->stream));
+    HIPC(hipMalloc(&h, bytes));
+    HIPC(hipMemsetAsync(h, 0, bytes, ctx->stream));
     if (ctx->hist) {
-        HIPC(hipMemcpyAsync(h, ctx->hist, (size_t)ctx->hist_cap * sizeof(double), hipMemcpyDeviceToDevice,
-                            ctx->stream));
+        HIPC(hipMemcpy2DAsync(h, (size_t)cap * sizeof(double), ctx->hist, (size_t)ctx->hist_cap * sizeof(double),
+                              (size_t)ctx->hist_cap * sizeof(double), ctx->n_inst, hipMemcpyDeviceToDevice,
+                              ctx->stream));
         HIPC(hipStreamSynchronize(ctx->stream));
         auto it = std::find(ctx->owned.begin(), ctx->owned.end(), (void *)ctx->hist);
         if (it != ctx->owned.end()) ctx->owned.erase(it);
         HIPC(hipFree(ctx->hist));
     }
+    HIPC(hipStreamSynchronize(ctx->stream));
     ctx->owned.push_back(h);
     ctx->hist = h;
     ctx->hist_cap = cap;
-    if (ctx->graph) {  // graph baked the old pointer
+    if (ctx->graph) {  // the graph holds the old pointer
         (void)hipGraphExecDestroy(ctx->graph);
         ctx->graph = nullptr;
     }
     return NF_OK;
 }
 
-// reference-order AoS [N][5] <-> device-order tiles [tile][5][64]
-int upload_aos(nf_ctx *ctx, const double *aos, double *dev) {
-    const uint32_t N = ctx->N;
-    std::vector<double> d((size_t)5 * ctx->npad, 0.0);
-    for (uint32_t c = 0; c < N; ++c)
-        for (int k = 0; k < 5; ++k)
-            d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)] = aos[5 * (size_t)ctx->perm[c] + k];
-    HIPC(hipMemcpy(dev, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+size_t tile_index(uint32_t c, int k) { return ((size_t)(c >> 6) * 5 + k) * TILE + (c & 63u); }
+
+// reference-order AoS [N][5] of instance i -> device tiles
+int upload_state(nf_ctx *ctx, int i, const double *aos, double *dev) {
+    const Inst &I = ctx->inst[i];
+    std::vector<double> d((size_t)5 * I.Npad, 0.0);
+    for (uint32_t c = 0; c < I.N; ++c)
+        for (int k = 0; k < 5; ++k) d[tile_index(c, k)] = aos[5 * (size_t)I.perm[c] + k];
+    HIPC(hipMemcpy(dev + (size_t)5 * I.cell0, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
     return NF_OK;
 }
 
-int download_tiled(nf_ctx *ctx, const double *dev, double *aos) {
-    const uint32_t N = ctx->N;
-    std::vector<double> d((size_t)5 * ctx->npad);
+// device tiles of instance i -> reference-order AoS [N][5]
+template <typename T>
+int download_state(nf_ctx *ctx, int i, const T *dev, double *aos) {
+    const Inst &I = ctx->inst[i];
+    std::vector<T> d((size_t)5 * I.Npad);
     HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t c = 0; c < N; ++c)
-        for (int k = 0; k < 5; ++k)
-            aos[5 * (size_t)ctx->perm[c] + k] = d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)];
+    HIPC(hipMemcpy(d.data(), dev + (size_t)5 * I.cell0, d.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < I.N; ++c)
+        for (int k = 0; k < 5; ++k) aos[5 * (size_t)I.perm[c] + k] = d[tile_index(c, k)];
     return NF_OK;
 }
 
-int download_tiled_f32(nf_ctx *ctx, const float *dev, double *aos) {
-    const uint32_t N = ctx->N;
-    std::vector<float> d((size_t)5 * ctx->npad);
+// device-cell AoS [cells][rows] of instance i -> reference-order AoS
+int download_aos(nf_ctx *ctx, int i, const double *dev, int rows, double *aos) {
+    const Inst &I = ctx->inst[i];
+    std::vector<double> d((size_t)rows * I.N);
     HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (uint32_t c = 0; c < N; ++c)
-        for (int k = 0; k < 5; ++k)
-            aos[5 * (size_t)ctx->perm[c] + k] = d[((size_t)(c >> 6) * 5 + k) * 64 + (c & 63u)];
+    HIPC(hipMemcpy(d.data(), dev + (size_t)rows * I.cell0, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < I.N; ++c)
+        std::memcpy(aos + (size_t)rows * I.perm[c], &d[(size_t)rows * c], rows * sizeof(double));
     return NF_OK;
 }
 
-// device-order AoS [N][rows] -> reference-order AoS
-int download_aos(nf_ctx *ctx, const double *dev, int rows, double *aos) {
-    const uint32_t N = ctx->N;
-    std::vector<double> d((size_t)rows * N);
-    HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(d.data(), dev, d.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t c = 0; c < N; ++c)
-        std::memcpy(aos + (size_t)rows * ctx->perm[c], &d[(size_t)rows * c], rows * sizeof(double));
-    return NF_OK;
-}
-
-int after_state_set(nf_ctx *ctx) {
+// primvars of instance i from its state, fresh run state
+int after_state_set(nf_ctx *ctx, int i, bool uold_zero) {
+    Inst &I = ctx->inst[i];
+    const dim3 g(I.Npad / BLOCK), b(BLOCK);
     if (ctx->mixed) {
-        HIPC(mx::launch_prim_mx(dim3((ctx->N + BLOCK - 1) / BLOCK), ctx->stream, ctx->Ua, ctx->V32, ctx->N));
+        HIPC(mx::launch_prim_mx(g, ctx->stream, ctx->Ua + (size_t)5 * I.cell0, ctx->V32 + (size_t)5 * I.cell0,
+                                I.Npad));
     } else {
-        hipLaunchKernelGGL(k_prim, dim3((ctx->N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua,
-                           ctx->V, ctx->N);
+        hipLaunchKernelGGL(k_prim, g, b, 0, ctx->stream, (const double *)(ctx->Ua + (size_t)5 * I.cell0),
+                           ctx->V + (size_t)5 * I.cell0, I.Npad);
         HIPC(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->iscal, 0, 8, ctx->scal, 0, 0.0, 1);
+    hipLaunchKernelGGL(k_reset_inst, dim3(1), dim3(64), 0, ctx->stream, ctx->st, i, (int)uold_zero);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(ctx->stream));
-    ctx->dt_valid = false;
+    I.dt_valid = false;
+    ctx->aux_valid = false;
+    return NF_OK;
+}
+
+// U_old of the reference's first RK3 step is all zeros unless the quirk is
+// switched off (Q1); a given U_old must equal U, as it does after any step
+int set_state(nf_ctx *ctx, int i, const double *U, const double *U_old) {
+    bool zero = ctx->prm.rk3_zero_old_first_step != 0;
+    if (U_old) {
+        if (std::memcmp(U, U_old, (size_t)5 * ctx->inst[i].N * sizeof(double)) != 0)
+            return fail(ctx, NF_E_ARG, "U_old must be NULL (zero) or equal to U");
+        zero = false;
+    }
+    int rc = upload_state(ctx, i, U, ctx->Ua);
+    if (rc) return rc;
+    return after_state_set(ctx, i, zero);
+}
+
+// set_freestream_values (src/SolverData.cpp:50-58): U = prim_to_cons(V_inf),
+// primvars = V_inf exactly
+int set_freestream(nf_ctx *ctx, int i) {
+    const Inst &I = ctx->inst[i];
+    double V[5], U[5];
+    std::memcpy(V, I.Vinf, sizeof V);
+    prim_to_cons(V, U);
+    std::vector<double> aos((size_t)5 * I.N);
+    for (uint32_t c = 0; c < I.N; ++c) std::memcpy(&aos[5 * (size_t)c], U, sizeof U);
+    int rc = upload_state(ctx, i, aos.data(), ctx->Ua);
+    if (rc) return rc;
+    if ((rc = after_state_set(ctx, i, ctx->prm.rk3_zero_old_first_step != 0))) return rc;
+    if (!ctx->mixed) {
+        for (uint32_t c = 0; c < I.N; ++c) std::memcpy(&aos[5 * (size_t)c], V, sizeof V);
+        if ((rc = upload_state(ctx, i, aos.data(), ctx->V))) return rc;
+    }
+    return NF_OK;
+}
+
+int build_red_table(nf_ctx *ctx, uint32_t B, RedTable &r) {
+    std::vector<uint32_t> first, n, c0, nc;
+    for (const Inst &I : ctx->inst) {
+        const uint32_t b0 = I.cell0 / B, nb = I.Npad / B;
+        c0.push_back((uint32_t)first.size());
+        for (uint32_t q = 0; q < nb; q += RCHUNK) {
+            first.push_back(b0 + q);
+            n.push_back(std::min(RCHUNK, nb - q));
+        }
+        nc.push_back((uint32_t)first.size() - c0.back());
+    }
+    r.nchunk = (int)first.size();
+    int rc;
+    if ((rc = dalloc(ctx, &r.first, first.size()))) return rc;
+    if ((rc = dalloc(ctx, &r.n, n.size()))) return rc;
+    if ((rc = dalloc(ctx, &r.inst_c0, c0.size()))) return rc;
+    if ((rc = dalloc(ctx, &r.inst_nc, nc.size()))) return rc;
+    HIPC(hipMemcpy(r.first, first.data(), first.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(r.n, n.data(), n.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(r.inst_c0, c0.data(), c0.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(r.inst_nc, nc.data(), nc.size() * 4, hipMemcpyHostToDevice));
+    return NF_OK;
+}
+
+int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx **out) {
+    if (!descs || !prms || !out || n < 1 || n > 65535) return NF_E_ARG;
+    *out = nullptr;
+    const nf_params &p0 = prms[0];
+    for (int i = 0; i < n; ++i) {
+        const nf_params &p = prms[i];
+        if (p.inv_flux_scheme != NF_FLUX_RUSANOV) return NF_E_ARG;  // HLLC: assert(false) in the reference
+        if (p.time_scheme != NF_TVD_RK3 && p.time_scheme != NF_EXPLICIT_EULER) return NF_E_ARG;
+        if (p.precision != NF_PREC_FP64 && p.precision != NF_PREC_MIXED) return NF_E_ARG;
+        // one launch serves all instances: everything but the freestream is shared
+        if (p.time_scheme != p0.time_scheme || p.spatial_order != p0.spatial_order || p.limiter != p0.limiter ||
+            p.CFL != p0.CFL || p.check_physical_validity != p0.check_physical_validity ||
+            p.rk3_zero_old_first_step != p0.rk3_zero_old_first_step || p.precision != p0.precision ||
+            p.cell_order != p0.cell_order || p.local_time_step != p0.local_time_step)
+            return NF_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NF_E_NOGPU;
+    std::unique_ptr<nf_ctx> hold(new nf_ctx);
+    nf_ctx *ctx = hold.get();
+    ctx->prm = p0;
+    ctx->n_inst = n;
+    ctx->mixed = p0.precision == NF_PREC_MIXED;
+    HIPC(hipGetDevice(&ctx->device));
+
+    // host pipeline per instance; the device arrays are the concatenation
+    std::vector<uint32_t> cnodes, nbr, fcode, bface, inst_end;
+    std::vector<uint8_t> bbc;
+    std::vector<uint16_t> binst, blk_inst;
+    std::vector<double> nodes4;
+    ctx->inst.resize(n);
+    uint64_t cell0 = 0, node0 = 0;
+    for (int i = 0; i < n; ++i) {
+        const nf_mesh_desc *d = &descs[i];
+        if (!d->nodes || !d->cells || d->n_cells == 0) return NF_E_MESH;
+        nf::PrimalMesh pm;
+        nf::FVGrid g;
+        nf::CellLayout L;
+        try {
+            pm.nodes.assign(d->nodes, d->nodes + 3 * (size_t)d->n_nodes);
+            pm.cells.assign(d->cells, d->cells + 4 * (size_t)d->n_cells);
+            pm.patch_n.assign(d->patch_n_faces, d->patch_n_faces + d->n_patches);
+            size_t nt = 0;
+            for (uint32_t q = 0; q < d->n_patches; ++q) nt += d->patch_n_faces[q];
+            pm.patch_tri.assign(d->patch_faces, d->patch_faces + 3 * nt);
+            pm.patch_bc.assign(d->patch_bc, d->patch_bc + d->n_patches);
+            for (int32_t b : pm.patch_bc)
+                if (b < 0 || b > 2) return NF_E_ARG;
+            nf::build_fv_grid(pm, g);
+            nf::build_cell_layout(pm, g, p0.cell_order, L);
+        } catch (const nf::MeshError &) {
+            return NF_E_MESH;
+        }
+        Inst &I = ctx->inst[i];
+        I.cell0 = (uint32_t)cell0;
+        I.N = g.N;
+        I.Npad = (g.N + BLOCK - 1) / BLOCK * BLOCK;
+        if (cell0 + I.Npad > NBR_INDEX_MASK || node0 + d->n_nodes > 0xFFFFFFFFull) return NF_E_ARG;
+        I.sizes = {g.N, g.NT, g.Fint, g.F, (uint32_t)g.pfirst.size()};
+        I.perm = L.perm;
+        std::memcpy(I.Vinf, prms[i].primvars_inf, sizeof I.Vinf);
+        const uint32_t b0 = (uint32_t)bface.size();
+        // padding cells repeat the last cell's topology; the sweeps never
+        // store to them and clamp their lanes to the instance's last cell
+        for (uint32_t c = 0; c < I.Npad; ++c) {
+            const uint32_t src = std::min(c, g.N - 1);
+            for (int q = 0; q < 4; ++q) {
+                cnodes.push_back(L.cnodes[4 * (size_t)src + q] + (uint32_t)node0);
+                const uint32_t w = L.nbr[4 * (size_t)src + q];
+                const uint32_t off = (w & NBR_BOUNDARY) ? b0 : (uint32_t)cell0;
+                nbr.push_back((w & ~NBR_INDEX_MASK) | ((w & NBR_INDEX_MASK) + off));
+            }
+            fcode.push_back(L.fcode[src]);
+        }
+        for (size_t q = 0; q < L.bface.size(); ++q) {
+            const uint32_t w = L.bface[q];
+            bface.push_back((w & ~NBR_INDEX_MASK) | ((w & NBR_INDEX_MASK) + (uint32_t)cell0));
+            bbc.push_back(L.bbc[q]);
+            binst.push_back((uint16_t)i);
+        }
+        for (uint32_t b = 0; b < I.Npad / BLOCK; ++b) blk_inst.push_back((uint16_t)i);
+        inst_end.push_back((uint32_t)cell0 + g.N);
+        for (uint32_t v = 0; v < d->n_nodes; ++v)
+            nodes4.insert(nodes4.end(),
+                          {d->nodes[3 * (size_t)v], d->nodes[3 * (size_t)v + 1], d->nodes[3 * (size_t)v + 2], 0.0});
+        cell0 += I.Npad;
+        node0 += d->n_nodes;
+    }
+    if (bface.size() > NBR_INDEX_MASK) return NF_E_ARG;
+    ctx->Ncell = (uint32_t)cell0;
+    ctx->nbf = (uint32_t)bface.size();
+
+    HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int rc;
+    const size_t NC = ctx->Ncell;
+    if ((rc = dalloc(ctx, &ctx->nodes, nodes4.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->cnodes, cnodes.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->nbr, nbr.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->fcode, fcode.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->blk_inst, blk_inst.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->inst_end, inst_end.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->st, (size_t)n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Ua, 5 * NC))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Ub, 5 * NC))) return rc;
+    if (ctx->mixed) {
+        if ((rc = dalloc(ctx, &ctx->V32, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV32, 20 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
+    } else {
+        if ((rc = dalloc(ctx, &ctx->V, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV, 20 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+    }
+    if (p0.local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NC))) return rc;
+    if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->binst, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->fs, (size_t)n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->fs32, (size_t)n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->dt_part, NC / 128))) return rc;
+    if ((rc = dalloc(ctx, &ctx->res_part, NC / 128))) return rc;
+    HIPC(hipMemset(ctx->Ua, 0, 5 * NC * sizeof(double)));
+    HIPC(hipMemset(ctx->Ub, 0, 5 * NC * sizeof(double)));
+    HIPC(hipMemcpy(ctx->nodes, nodes4.data(), nodes4.size() * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->cnodes, cnodes.data(), cnodes.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->fcode, fcode.data(), fcode.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->blk_inst, blk_inst.data(), blk_inst.size() * 2, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->inst_end, inst_end.data(), inst_end.size() * 4, hipMemcpyHostToDevice));
+    if (ctx->nbf) {
+        HIPC(hipMemcpy(ctx->bface, bface.data(), bface.size() * 4, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(ctx->bbc, bbc.data(), bbc.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(ctx->binst, binst.data(), binst.size() * 2, hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<FarField<double>> f64(n);
+        std::vector<FarField<float>> f32(n);
+        for (int i = 0; i < n; ++i) {
+            f64[i] = make_farfield<double>(prms[i].primvars_inf);
+            f32[i] = make_farfield<float>(prms[i].primvars_inf);
+        }
+        HIPC(hipMemcpy(ctx->fs, f64.data(), n * sizeof(FarField<double>), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(ctx->fs32, f32.data(), n * sizeof(FarField<float>), hipMemcpyHostToDevice));
+    }
+    if ((rc = build_red_table(ctx, 256, ctx->red256))) return rc;
+    if ((rc = build_red_table(ctx, 128, ctx->red128))) return rc;
+    if ((rc = dalloc(ctx, &ctx->red_dt, (size_t)ctx->red128.nchunk))) return rc;
+    if ((rc = dalloc(ctx, &ctx->red_res, (size_t)ctx->red128.nchunk))) return rc;
+    if ((rc = ensure_hist(ctx, 1024))) return rc;
+    for (int i = 0; i < n; ++i)
+        if ((rc = set_freestream(ctx, i))) return rc;
+    *out = hold.release();
+    return NF_OK;
+}
+
+bool single(const nf_ctx *ctx) { return ctx && ctx->n_inst == 1; }
+bool valid_inst(const nf_ctx *ctx, int32_t i) { return ctx && i >= 0 && i < ctx->n_inst; }
+
+// one step of a single-instance context with the dt held on the device
+int step_device_dt(nf_ctx *ctx) {
+    int rc;
+    if ((rc = clear_flags(ctx))) return rc;
+    std::vector<InstState> h;
+    if ((rc = read_states(ctx, h))) return rc;
+    if ((rc = ensure_hist(ctx, h[0].nstep + 2))) return rc;
+    if ((rc = enqueue_step(ctx, -1.0))) return rc;
+    if (ctx->timing && (rc = flush_timing(ctx))) return rc;
+    ctx->inst[0].dt_valid = true;
+    ctx->aux_valid = false;
+    return check_errors(ctx);
+}
+
+// device-resident loop over all instances, K steps per graph launch; the host
+// reads the stop flags between launches only when a residual drop is set
+int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
+    if (!o || o->n_steps < 0) return NF_E_ARG;
+    int rc;
+    const int K = std::max(1, o->check_every);
+    if ((rc = clear_flags(ctx))) return rc;
+    bool need_dt = false;
+    for (const Inst &I : ctx->inst) need_dt |= !I.dt_valid;
+    if (need_dt && (rc = enqueue_timestep(ctx))) return rc;
+    for (Inst &I : ctx->inst) I.dt_valid = true;
+    std::vector<InstState> h0;
+    if ((rc = read_states(ctx, h0))) return rc;
+    int maxstep = 0;
+    for (auto &s : h0) maxstep = std::max(maxstep, s.nstep);
+    if ((rc = ensure_hist(ctx, maxstep + o->n_steps + 2))) return rc;
+    const double drop = o->residual_drop;
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, ctx->stream));
+    int done = 0;
+    bool stopped = false;
+    while (done < o->n_steps && !stopped) {
+        const int chunk = std::min(K, o->n_steps - done);
+        if (o->use_graph && chunk == K) {
+            if (!ctx->graph || ctx->graph_steps != K || ctx->graph_drop != drop || ctx->graph_block != ctx->block ||
+                ctx->graph_xcd != ctx->xcd_remap) {
+                if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+                ctx->graph = nullptr;
+                hipGraph_t gr;
+                HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                for (int s = 0; s < K; ++s)
+                    if ((rc = enqueue_step(ctx, drop))) {
+                        (void)hipStreamEndCapture(ctx->stream, &gr);
+                        return rc;
+                    }
+                HIPC(hipStreamEndCapture(ctx->stream, &gr));
+                HIPC(hipGraphInstantiate(&ctx->graph, gr, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(gr);
+                ctx->graph_steps = K;
+                ctx->graph_drop = drop;
+                ctx->graph_block = ctx->block;
+                ctx->graph_xcd = ctx->xcd_remap;
+            }
+            HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
+        } else {
+            for (int s = 0; s < chunk; ++s)
+                if ((rc = enqueue_step(ctx, drop))) return rc;
+        }
+        done += chunk;
+        if (drop > 0.0 && done < o->n_steps) {
+            std::vector<InstState> h;
+            if ((rc = read_states(ctx, h))) return rc;
+            stopped = true;
+            for (auto &s : h) stopped &= s.stop != 0;
+        }
+    }
+    HIPC(hipEventRecord(e1, ctx->stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->aux_valid = false;
+    std::vector<InstState> h;
+    if ((rc = read_states(ctx, h))) return rc;
+    if (stats) {
+        for (int i = 0; i < ctx->n_inst; ++i) {
+            nf_run_stats &s = stats[i];
+            s.steps_done = h[i].nstep - h0[i].nstep;
+            s.converged = h[i].converged;
+            s.time = h[i].time;
+            s.last_dt = h[i].dt;
+            s.elapsed_ms = ms;
+            double r[2] = {0, 0};
+            const double *hp = ctx->hist + (size_t)i * ctx->hist_cap;
+            HIPC(hipMemcpy(&r[0], hp, sizeof(double), hipMemcpyDeviceToHost));
+            if (h[i].nstep > 0)
+                HIPC(hipMemcpy(&r[1], hp + std::min(h[i].nstep - 1, ctx->hist_cap - 1), sizeof(double),
+                               hipMemcpyDeviceToHost));
+            s.res_first = r[0];
+            s.res_last = r[1];
+        }
+    }
+    return check_errors(ctx);
+}
+
+// saved copy of the state and run counters (autotuning runs on the copy)
+struct Snapshot {
+    std::vector<double> U, V;
+    std::vector<float> V32;
+    std::vector<InstState> st;
+    std::vector<bool> dt_valid;
+};
+
+int save(nf_ctx *ctx, Snapshot &s) {
+    const size_t n5 = 5 * (size_t)ctx->Ncell;
+    int rc;
+    if ((rc = read_states(ctx, s.st))) return rc;
+    s.U.resize(n5);
+    HIPC(hipMemcpy(s.U.data(), ctx->Ua, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    if (ctx->mixed) {
+        s.V32.resize(n5);
+        HIPC(hipMemcpy(s.V32.data(), ctx->V32, n5 * sizeof(float), hipMemcpyDeviceToHost));
+    } else {
+        s.V.resize(n5);
+        HIPC(hipMemcpy(s.V.data(), ctx->V, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    s.dt_valid.clear();
+    for (auto &I : ctx->inst) s.dt_valid.push_back(I.dt_valid);
+    return NF_OK;
+}
+
+int restore(nf_ctx *ctx, const Snapshot &s) {
+    const size_t n5 = 5 * (size_t)ctx->Ncell;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(ctx->Ua, s.U.data(), n5 * sizeof(double), hipMemcpyHostToDevice));
+    if (ctx->mixed) HIPC(hipMemcpy(ctx->V32, s.V32.data(), n5 * sizeof(float), hipMemcpyHostToDevice));
+    else HIPC(hipMemcpy(ctx->V, s.V.data(), n5 * sizeof(double), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->st, s.st.data(), s.st.size() * sizeof(InstState), hipMemcpyHostToDevice));
+    for (size_t i = 0; i < ctx->inst.size(); ++i) ctx->inst[i].dt_valid = s.dt_valid[i];
     ctx->aux_valid = false;
     return NF_OK;
 }
@@ -470,97 +850,10 @@ int after_state_set(nf_ctx *ctx) {
 // ============================================================================
 extern "C" {
 
-int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) {
-    if (!desc || !prm || !out) return NF_E_ARG;
-    *out = nullptr;
-    if (prm->inv_flux_scheme != NF_FLUX_RUSANOV) return NF_E_ARG;  // HLLC: assert(false) in the reference
-    if (prm->time_scheme != NF_TVD_RK3 && prm->time_scheme != NF_EXPLICIT_EULER) return NF_E_ARG;
-    if (prm->precision != NF_PREC_FP64 && prm->precision != NF_PREC_MIXED) return NF_E_ARG;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NF_E_NOGPU;
-    std::unique_ptr<nf_ctx> hold(new nf_ctx);
-    nf_ctx *ctx = hold.get();
-    ctx->prm = *prm;
-    HIPC(hipGetDevice(&ctx->device));
-    nf::PrimalMesh pm;
-    nf::FVGrid g;
-    nf::CellLayout L;
-    try {
-        pm.nodes.assign(desc->nodes, desc->nodes + 3 * (size_t)desc->n_nodes);
-        pm.cells.assign(desc->cells, desc->cells + 4 * (size_t)desc->n_cells);
-        pm.patch_n.assign(desc->patch_n_faces, desc->patch_n_faces + desc->n_patches);
-        size_t nt = 0;
-        for (uint32_t p = 0; p < desc->n_patches; ++p) nt += desc->patch_n_faces[p];
-        pm.patch_tri.assign(desc->patch_faces, desc->patch_faces + 3 * nt);
-        pm.patch_bc.assign(desc->patch_bc, desc->patch_bc + desc->n_patches);
-        for (int32_t b : pm.patch_bc)
-            if (b < 0 || b > 2) return NF_E_ARG;
-        nf::build_fv_grid(pm, g);
-        nf::build_cell_layout(pm, g, prm->cell_order, L);
-    } catch (const nf::MeshError &e) {
-        return NF_E_MESH;
-    }
-    ctx->N = g.N;
-    ctx->n_nodes = desc->n_nodes;
-    ctx->sizes = {g.N, g.NT, g.Fint, g.F, (uint32_t)g.pfirst.size()};
-    ctx->perm = L.perm;
-    ctx->iperm = L.iperm;
-    ctx->fs = make_farfield<double>(prm->primvars_inf);
-    ctx->fs32 = make_farfield<float>(prm->primvars_inf);
-    ctx->mixed = prm->precision == NF_PREC_MIXED;
-    ctx->nb = (int)((g.N + BLOCK - 1) / BLOCK);
-    ctx->npad = (g.N + TILE - 1) / TILE * TILE;
-    HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    const uint32_t N = g.N;
-    int rc;
-    if ((rc = dalloc(ctx, &ctx->nodes, 4 * (size_t)ctx->n_nodes))) return rc;
-    if ((rc = dalloc(ctx, &ctx->cnodes, 4 * (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->nbr, 4 * (size_t)N))) return rc;
-    if ((rc = dalloc(ctx, &ctx->fcode, (size_t)N))) return rc;
-    const size_t NP = ctx->npad;
-    if ((rc = dalloc(ctx, &ctx->Ua, 5 * NP))) return rc;
-    if ((rc = dalloc(ctx, &ctx->Ub, 5 * NP))) return rc;
-    if (ctx->mixed) {
-        if ((rc = dalloc(ctx, &ctx->V32, 5 * NP))) return rc;
-        if ((rc = dalloc(ctx, &ctx->FV32, 20 * NP))) return rc;
-    } else {
-        if ((rc = dalloc(ctx, &ctx->V, 5 * NP))) return rc;
-        if ((rc = dalloc(ctx, &ctx->FV, 20 * NP))) return rc;
-    }
-    if (prm->local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NP))) return rc;
-    ctx->nbf = (uint32_t)L.bface.size();
-    if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
-    if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
-    if (ctx->mixed) {
-        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
-    } else {
-        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
-    }
-    if (ctx->nbf) {
-        HIPC(hipMemcpy(ctx->bface, L.bface.data(), L.bface.size() * 4, hipMemcpyHostToDevice));
-        HIPC(hipMemcpy(ctx->bbc, L.bbc.data(), L.bbc.size(), hipMemcpyHostToDevice));
-    }
-    if ((rc = dalloc(ctx, &ctx->dt_part, (size_t)ctx->nb))) return rc;
-    if ((rc = dalloc(ctx, &ctx->res_part, (size_t)ctx->nb))) return rc;
-    ctx->nb2 = (ctx->nb + RCHUNK - 1) / RCHUNK;
-    if ((rc = dalloc(ctx, &ctx->red2, 2 * (size_t)ctx->nb2))) return rc;
-    if ((rc = dalloc(ctx, &ctx->scal, 4))) return rc;
-    if ((rc = dalloc(ctx, &ctx->iscal, 8))) return rc;
-    {
-        std::vector<double> n4((size_t)4 * ctx->n_nodes, 0.0);
-        for (uint32_t n = 0; n < ctx->n_nodes; ++n)
-            for (int d = 0; d < 3; ++d) n4[4 * (size_t)n + d] = desc->nodes[3 * (size_t)n + d];
-        HIPC(hipMemcpy(ctx->nodes, n4.data(), n4.size() * sizeof(double), hipMemcpyHostToDevice));
-    }
-    HIPC(hipMemcpy(ctx->cnodes, L.cnodes.data(), L.cnodes.size() * 4, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(ctx->nbr, L.nbr.data(), L.nbr.size() * 4, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(ctx->fcode, L.fcode.data(), L.fcode.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = ensure_hist(ctx, 1024))) return rc;
-    if ((rc = nf_set_freestream(ctx))) return rc;
-    *out = hold.release();
-    return NF_OK;
+int nf_create(const nf_mesh_desc *desc, const nf_params *prm, nf_ctx **out) { return create(1, desc, prm, out); }
+
+int nf_create_batch(int32_t n, const nf_mesh_desc *meshes, const nf_params *params, nf_ctx **out) {
+    return create(n, meshes, params, out);
 }
 
 int nf_destroy(nf_ctx *ctx) {
@@ -577,59 +870,52 @@ const char *nf_last_error(const nf_ctx *ctx) { return ctx ? ctx->err.c_str() : "
 
 int nf_ctx_sizes(const nf_ctx *ctx, nf_fv_sizes *s) {
     if (!ctx || !s) return NF_E_ARG;
-    *s = ctx->sizes;
+    *s = ctx->inst[0].sizes;
+    return NF_OK;
+}
+
+int nf_ctx_instances(const nf_ctx *ctx, int32_t *n) {
+    if (!ctx || !n) return NF_E_ARG;
+    *n = ctx->n_inst;
+    return NF_OK;
+}
+
+int nf_inst_sizes(const nf_ctx *ctx, int32_t i, nf_fv_sizes *s) {
+    if (!valid_inst(ctx, i) || !s) return NF_E_ARG;
+    *s = ctx->inst[i].sizes;
     return NF_OK;
 }
 
 int nf_set_freestream(nf_ctx *ctx) {
     if (!ctx) return NF_E_ARG;
-    double V[5], U[5];
-    for (int k = 0; k < 5; ++k) V[k] = ctx->prm.primvars_inf[k];
-    prim_to_cons(V, U);
-    std::vector<double> aos((size_t)5 * ctx->N);
-    for (uint32_t c = 0; c < ctx->N; ++c)
-        for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = U[k];
-    int rc = upload_aos(ctx, aos.data(), ctx->Ua);
-    if (rc) return rc;
-    if ((rc = after_state_set(ctx))) return rc;
-    // primvars = V_inf exactly (set_freestream_values, src/SolverData.cpp:50-58)
-    if (!ctx->mixed) {
-        for (uint32_t c = 0; c < ctx->N; ++c)
-            for (int k = 0; k < 5; ++k) aos[5 * (size_t)c + k] = V[k];
-        if ((rc = upload_aos(ctx, aos.data(), ctx->V))) return rc;
+    for (int i = 0; i < ctx->n_inst; ++i) {
+        int rc = set_freestream(ctx, i);
+        if (rc) return rc;
     }
-    ctx->uold_zero = true;
     return NF_OK;
 }
 
 int nf_set_state(nf_ctx *ctx, const double *U, const double *U_old) {
-    if (!ctx || !U) return NF_E_ARG;
-    int rc = upload_aos(ctx, U, ctx->Ua);
-    if (rc) return rc;
-    if (U_old) {
-        // U_old is held in Ua between steps: a non-zero U_old different from U
-        // can only be represented for the next step's stages 2/3 via Ub staging
-        std::vector<double> d((size_t)5 * ctx->N);
-        bool same = std::memcmp(U, U_old, d.size() * sizeof(double)) == 0;
-        if (!same) return fail(ctx, NF_E_ARG, "U_old must be NULL (zero) or equal to U");
-        ctx->uold_zero = false;
-    } else {
-        ctx->uold_zero = true;
-    }
-    return after_state_set(ctx);
+    if (!single(ctx) || !U) return NF_E_ARG;
+    return set_state(ctx, 0, U, U_old);
+}
+
+int nf_inst_set_state(nf_ctx *ctx, int32_t i, const double *U, const double *U_old) {
+    if (!valid_inst(ctx, i) || !U) return NF_E_ARG;
+    return set_state(ctx, i, U, U_old);
 }
 
 int nf_calc_timestep(nf_ctx *ctx, double *dt_out) {
-    if (!ctx) return NF_E_ARG;
+    if (!single(ctx)) return NF_E_ARG;
     int rc;
-    if (!ctx->dt_valid) {
+    if (!ctx->inst[0].dt_valid) {
         if ((rc = clear_flags(ctx))) return rc;
         if ((rc = enqueue_timestep(ctx))) return rc;
-        ctx->dt_valid = true;
+        ctx->inst[0].dt_valid = true;
     }
-    double dt = 0;
-    HIPC(hipMemcpyAsync(&dt, ctx->scal, sizeof dt, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
+    std::vector<InstState> h;
+    if ((rc = read_states(ctx, h))) return rc;
+    const double dt = h[0].dt;
     if (dt_out) *dt_out = dt;
     if (!((dt - dt == 0.0) && dt > 0.0)) {
         char buf[96];
@@ -639,191 +925,109 @@ int nf_calc_timestep(nf_ctx *ctx, double *dt_out) {
     return NF_OK;
 }
 
-}  // extern "C"
-
-// One step with the dt currently held on the device.
-static int step_device_dt(nf_ctx *ctx) {
-    int rc;
-    if ((rc = clear_flags(ctx))) return rc;
-    int nstep = 0;
-    HIPC(hipMemcpyAsync(&nstep, ctx->iscal, sizeof nstep, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
-    if ((rc = ensure_hist(ctx, nstep + 2))) return rc;
-    const bool zero_old = ctx->uold_zero && ctx->prm.rk3_zero_old_first_step;
-    if ((rc = enqueue_step(ctx, zero_old, -1.0, false))) return rc;
-    if (ctx->timing && (rc = flush_timing(ctx))) return rc;
-    ctx->uold_zero = false;
-    ctx->dt_valid = true;
-    ctx->aux_valid = false;
-    return check_errors(ctx);
-}
-
-extern "C" {
-
 int nf_step(nf_ctx *ctx, double dt) {
-    if (!ctx) return NF_E_ARG;
-    hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->iscal, 1, 0, ctx->scal, 1, dt, 0);
+    if (!single(ctx)) return NF_E_ARG;
+    hipLaunchKernelGGL(k_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->st, 1, 1, 0, dt);
     HIPC(hipGetLastError());
     return step_device_dt(ctx);
 }
 
 int nf_run(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *st) {
-    if (!ctx || !o || o->n_steps < 0) return NF_E_ARG;
-    int rc;
-    const int K = std::max(1, o->check_every);
-    if ((rc = clear_flags(ctx))) return rc;
-    if (!ctx->dt_valid) {
-        if ((rc = enqueue_timestep(ctx))) return rc;
-        ctx->dt_valid = true;
-    }
-    int n0 = 0;
-    HIPC(hipMemcpyAsync(&n0, ctx->iscal, sizeof n0, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
-    if ((rc = ensure_hist(ctx, n0 + o->n_steps + 2))) return rc;
-    const double drop = o->residual_drop;
-    hipEvent_t e0, e1;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
-    HIPC(hipEventRecord(e0, ctx->stream));
-    int done = 0;
-    bool stopped = false;
-    if (done < o->n_steps && ctx->uold_zero && ctx->prm.rk3_zero_old_first_step) {
-        if ((rc = enqueue_step(ctx, true, drop, true))) return rc;
-        ctx->uold_zero = false;
-        ++done;
-    }
-    ctx->uold_zero = false;
-    while (done < o->n_steps && !stopped) {
-        const int chunk = std::min(K, o->n_steps - done);
-        if (o->use_graph && chunk == K) {
-            if (!ctx->graph || ctx->graph_steps != K || ctx->graph_drop != drop) {
-                if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
-                ctx->graph = nullptr;
-                hipGraph_t gr;
-                HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                for (int s = 0; s < K; ++s)
-                    if ((rc = enqueue_step(ctx, false, drop, true))) {
-                        (void)hipStreamEndCapture(ctx->stream, &gr);
-                        return rc;
-                    }
-                HIPC(hipStreamEndCapture(ctx->stream, &gr));
-                HIPC(hipGraphInstantiate(&ctx->graph, gr, nullptr, nullptr, 0));
-                (void)hipGraphDestroy(gr);
-                ctx->graph_steps = K;
-                ctx->graph_drop = drop;
-            }
-            HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
-        } else {
-            for (int s = 0; s < chunk; ++s)
-                if ((rc = enqueue_step(ctx, false, drop, true))) return rc;
-        }
-        done += chunk;
-        if (drop > 0.0 || done >= o->n_steps) {
-            int is[4];
-            HIPC(hipMemcpyAsync(is, ctx->iscal, sizeof is, hipMemcpyDeviceToHost, ctx->stream));
-            HIPC(hipStreamSynchronize(ctx->stream));
-            if (is[2]) stopped = true;
-        }
-    }
-    HIPC(hipEventRecord(e1, ctx->stream));
-    HIPC(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPC(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    ctx->aux_valid = false;
-    int is[6];
-    double sc[2];
-    HIPC(hipMemcpy(is, ctx->iscal, sizeof is, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(sc, ctx->scal, sizeof sc, hipMemcpyDeviceToHost));
-    if (st) {
-        st->steps_done = is[0] - n0;
-        st->converged = is[3];
-        st->time = sc[1];
-        st->last_dt = sc[0];
-        st->elapsed_ms = ms;
-        double r[2] = {0, 0};
-        HIPC(hipMemcpy(&r[0], ctx->hist, sizeof(double), hipMemcpyDeviceToHost));
-        if (is[0] > 0)
-            HIPC(hipMemcpy(&r[1], ctx->hist + std::min(is[0] - 1, ctx->hist_cap - 1), sizeof(double),
-                           hipMemcpyDeviceToHost));
-        st->res_first = r[0];
-        st->res_last = r[1];
-    }
-    return check_errors(ctx);
+    if (!single(ctx)) return NF_E_ARG;
+    return run_loop(ctx, o, st);
 }
 
-int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
-    if (!ctx || !out) return NF_E_ARG;
-    int n = 0;
-    HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemcpy(&n, ctx->iscal, sizeof n, hipMemcpyDeviceToHost));
-    n = std::min(std::min(n, max), ctx->hist_cap);
-    if (n > 0) HIPC(hipMemcpy(out, ctx->hist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-    if (n_out) *n_out = n;
+int nf_run_batch(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
+    if (!ctx) return NF_E_ARG;
+    return run_loop(ctx, o, stats);
+}
+
+int nf_inst_residuals(nf_ctx *ctx, int32_t i, double *out, int32_t max, int32_t *n_out) {
+    if (!valid_inst(ctx, i) || !out) return NF_E_ARG;
+    std::vector<InstState> h;
+    int rc;
+    if ((rc = read_states(ctx, h))) return rc;
+    const int n = std::min(std::min(h[i].nstep, max), ctx->hist_cap);
+    if (n > 0)
+        HIPC(hipMemcpy(out, ctx->hist + (size_t)i * ctx->hist_cap, (size_t)n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    if (n_out) *n_out = std::max(n, 0);
     return NF_OK;
 }
 
+int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
+    if (!single(ctx)) return NF_E_ARG;
+    return nf_inst_residuals(ctx, 0, out, max, n_out);
+}
+
 int nf_get_state(nf_ctx *ctx, double *U) {
-    if (!ctx || !U) return NF_E_ARG;
-    return download_tiled(ctx, ctx->Ua, U);
+    if (!single(ctx) || !U) return NF_E_ARG;
+    return download_state(ctx, 0, (const double *)ctx->Ua, U);
+}
+
+int nf_inst_get_state(nf_ctx *ctx, int32_t i, double *U) {
+    if (!valid_inst(ctx, i) || !U) return NF_E_ARG;
+    return download_state(ctx, i, (const double *)ctx->Ua, U);
 }
 
 // Evaluates the flux balance of the current state once (Solver::
 // evaluate_flux_balance, src/Solver.cpp:54-81) without updating U, keeping
 // gradient, limiter and R for nf_get_field.
 int nf_eval_flux_balance(nf_ctx *ctx) {
-    if (!ctx) return NF_E_ARG;
+    if (!single(ctx)) return NF_E_ARG;
     int rc;
-    const uint32_t N = ctx->N;
+    const size_t NC = ctx->Ncell;
     if (!ctx->R_aux) {
-        if ((rc = dalloc(ctx, &ctx->grad_aux, 15 * (size_t)N))) return rc;
-        if ((rc = dalloc(ctx, &ctx->psi_aux, 5 * (size_t)N))) return rc;
-        if ((rc = dalloc(ctx, &ctx->R_aux, 5 * (size_t)N))) return rc;
+        if ((rc = dalloc(ctx, &ctx->grad_aux, 15 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->psi_aux, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->R_aux, 5 * NC))) return rc;
     }
     if ((rc = clear_flags(ctx))) return rc;
-    // primvars of the current state
     if (ctx->mixed) {
-        HIPC(mx::launch_prim_mx(dim3((N + BLOCK - 1) / BLOCK), ctx->stream, ctx->Ua, ctx->V32, N));
+        HIPC(mx::launch_prim_mx(dim3(ctx->Ncell / BLOCK), ctx->stream, ctx->Ua, ctx->V32, ctx->Ncell));
     } else {
-        hipLaunchKernelGGL(k_prim, dim3((N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, ctx->Ua, ctx->V, N);
+        hipLaunchKernelGGL(k_prim, dim3(ctx->Ncell / BLOCK), dim3(BLOCK), 0, ctx->stream, (const double *)ctx->Ua,
+                           ctx->V, ctx->Ncell);
         HIPC(hipGetLastError());
     }
     if ((rc = run_sweepA(ctx, false, true))) return rc;
-    if ((rc = run_sweepB(ctx, 1, ctx->Ua, nullptr, ctx->Ub, false, true))) return rc;
+    if ((rc = run_sweepB(ctx, 1, ctx->Ua, ctx->Ub, false, true))) return rc;
     ctx->aux_valid = true;
     return check_errors(ctx);
 }
 
 int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
-    if (!ctx || !out) return NF_E_ARG;
+    if (!single(ctx) || !out) return NF_E_ARG;
     switch (which) {
     case NF_FIELD_PRIM:
-        if (ctx->mixed) return download_tiled_f32(ctx, ctx->V32, out);
-        return download_tiled(ctx, ctx->V, out);
+        if (ctx->mixed) return download_state(ctx, 0, (const float *)ctx->V32, out);
+        return download_state(ctx, 0, (const double *)ctx->V, out);
     case NF_FIELD_STATE_OLD: {
-        if (ctx->uold_zero) {
-            std::memset(out, 0, sizeof(double) * 5 * ctx->N);
+        std::vector<InstState> h;
+        int rc;
+        if ((rc = read_states(ctx, h))) return rc;
+        if (h[0].uold_zero) {
+            std::memset(out, 0, sizeof(double) * 5 * ctx->inst[0].N);
             return NF_OK;
         }
-        return download_tiled(ctx, ctx->Ua, out);
+        return download_state(ctx, 0, (const double *)ctx->Ua, out);
     }
     case NF_FIELD_LIMITER:
     case NF_FIELD_GRADIENT:
     case NF_FIELD_RESIDUAL:
         if (!ctx->aux_valid) return fail(ctx, NF_E_ARG, "call nf_eval_flux_balance first");
-        if (which == NF_FIELD_LIMITER) return download_aos(ctx, ctx->psi_aux, 5, out);
-        if (which == NF_FIELD_GRADIENT) return download_aos(ctx, ctx->grad_aux, 15, out);
-        return download_aos(ctx, ctx->R_aux, 5, out);
+        if (which == NF_FIELD_LIMITER) return download_aos(ctx, 0, ctx->psi_aux, 5, out);
+        if (which == NF_FIELD_GRADIENT) return download_aos(ctx, 0, ctx->grad_aux, 15, out);
+        return download_aos(ctx, 0, ctx->R_aux, 5, out);
     default:
         return fail(ctx, NF_E_ARG, "unknown field");
     }
 }
 
 int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out, int32_t max, int32_t *n_out) {
-    if (!ctx || n_steps < 1) return NF_E_ARG;
+    if (!single(ctx) || n_steps < 1 || !out) return NF_E_ARG;
     int rc;
-    if (!ctx->dt_valid) {
+    if (!ctx->inst[0].dt_valid) {
         double dt;
         if ((rc = nf_calc_timestep(ctx, &dt))) return rc;
     }
@@ -845,6 +1049,56 @@ int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out, int32_t m
         ++n;
     }
     if (n_out) *n_out = n;
+    return NF_OK;
+}
+
+int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap) {
+    if (!ctx || (block != 128 && block != 256)) return NF_E_ARG;
+    ctx->block = block;
+    ctx->xcd_remap = xcd_remap ? 1 : 0;
+    return NF_OK;
+}
+
+// Times every launch configuration (sweep block 256/128 x XCD remap on/off)
+// over `steps` graph-launched steps of the current state, keeps the fastest,
+// and restores the state, run counters and dt afterwards.  Results do not
+// depend on the configuration.
+int nf_autotune(nf_ctx *ctx, int32_t steps, nf_tune_result *out) {
+    if (!ctx || steps < 1) return NF_E_ARG;
+    int rc;
+    Snapshot snap;
+    if ((rc = save(ctx, snap))) return rc;
+    static const int cand[4][2] = {{256, 1}, {256, 0}, {128, 1}, {128, 0}};
+    nf_tune_result r{};
+    r.ms_per_step = 1e300;
+    r.block = ctx->block;
+    r.xcd_remap = ctx->xcd_remap;
+    for (int q = 0; q < 4; ++q) {
+        ctx->block = cand[q][0];
+        ctx->xcd_remap = cand[q][1];
+        nf_run_opts o{steps, 0.0, steps, 1};
+        std::vector<nf_run_stats> s(ctx->n_inst);
+        // the first run builds the graph and warms up; the second is timed
+        for (int pass = 0; pass < 2; ++pass) {
+            if ((rc = restore(ctx, snap))) return rc;
+            if ((rc = run_loop(ctx, &o, s.data()))) {
+                (void)restore(ctx, snap);
+                return rc;
+            }
+        }
+        const double ms = s[0].elapsed_ms / steps;
+        r.tried_ms[q] = ms;
+        if (ms < r.ms_per_step) {
+            r.ms_per_step = ms;
+            r.block = cand[q][0];
+            r.xcd_remap = cand[q][1];
+        }
+    }
+    r.n_tried = 4;
+    ctx->block = r.block;
+    ctx->xcd_remap = r.xcd_remap;
+    if ((rc = restore(ctx, snap))) return rc;
+    if (out) *out = r;
     return NF_OK;
 }
 

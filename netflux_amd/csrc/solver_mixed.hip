@@ -14,15 +14,22 @@ hipError_t launch_ghost_mx(bool face, dim3 g, hipStream_t st, const DevMesh &m, 
     return hipGetLastError();
 }
 
-hipError_t launch_sweepA_mx(int order, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a) {
-    if (order == 0) hipLaunchKernelGGL(k_sweepA_mx<0>, g, dim3(BLOCK), 0, st, m, a);
-    else if (order == 1) hipLaunchKernelGGL(k_sweepA_mx<1>, g, dim3(BLOCK), 0, st, m, a);
-    else hipLaunchKernelGGL(k_sweepA_mx<2>, g, dim3(BLOCK), 0, st, m, a);
+template <int B>
+static void sweepA_b(int order, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a) {
+    if (order == 0) hipLaunchKernelGGL((k_sweepA_mx<0, B>), g, dim3(B), 0, st, m, a);
+    else if (order == 1) hipLaunchKernelGGL((k_sweepA_mx<1, B>), g, dim3(B), 0, st, m, a);
+    else hipLaunchKernelGGL((k_sweepA_mx<2, B>), g, dim3(B), 0, st, m, a);
+}
+
+hipError_t launch_sweepA_mx(int order, int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a) {
+    if (block == 128) sweepA_b<128>(order, g, st, m, a);
+    else sweepA_b<256>(order, g, st, m, a);
     return hipGetLastError();
 }
 
-hipError_t launch_sweepB_mx(dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a) {
-    hipLaunchKernelGGL(k_sweepB_mx, g, dim3(BLOCK), 0, st, m, a);
+hipError_t launch_sweepB_mx(int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a) {
+    if (block == 128) hipLaunchKernelGGL(k_sweepB_mx<128>, g, dim3(128), 0, st, m, a);
+    else hipLaunchKernelGGL(k_sweepB_mx<256>, g, dim3(256), 0, st, m, a);
     return hipGetLastError();
 }
 

@@ -270,6 +270,90 @@ class EulerSolver:
         _check(A.lib().nf_time_kernels(self._ctx, n_steps, arr, max_k, C.byref(n)), self._ctx, "nf_time_kernels")
         return {arr[i].name.decode(): (arr[i].avg_ms, arr[i].calls) for i in range(n.value)}
 
+    def autotune(self, steps=10):
+        return _autotune(self._ctx, steps)
+
+    def set_launch(self, block, xcd_remap=True):
+        _check(A.lib().nf_set_launch(self._ctx, block, int(xcd_remap)), self._ctx, "nf_set_launch")
+
+
+def _autotune(ctx, steps):
+    r = A.TuneResult()
+    _check(A.lib().nf_autotune(ctx, steps, C.byref(r)), ctx, "nf_autotune")
+    return {"block": r.block, "xcd_remap": r.xcd_remap, "ms_per_step": r.ms_per_step,
+            "tried_ms": list(r.tried_ms[: r.n_tried])}
+
+
+class BatchSolver:
+    """Several meshes advanced together: every sweep is one launch over the
+    concatenated cells of all instances, each instance keeping its own time
+    step, step counter, residual history and stop flag.
+
+    configs may be one Config for all meshes or one per mesh; only the
+    freestream state may differ between them.
+    """
+
+    def __init__(self, configs, meshes):
+        meshes = list(meshes)
+        if isinstance(configs, Config):
+            configs = [configs] * len(meshes)
+        configs = list(configs)
+        if len(configs) != len(meshes) or not meshes:
+            raise ValueError("one config per mesh")
+        self.meshes = meshes
+        n = len(meshes)
+        descs = (A.MeshDesc * n)(*[m.desc for m in meshes])
+        params = (A.Params * n)(*[c.to_params() for c in configs])
+        ctx = C.c_void_p()
+        rc = A.lib().nf_create_batch(n, descs, params, C.byref(ctx))
+        if rc != A.NF_OK:
+            raise RuntimeError(f"nf_create_batch failed with status {rc}")
+        self._ctx = ctx
+        self.n = n
+        self.N = []
+        for i in range(n):
+            s = A.FVSizes()
+            _check(A.lib().nf_inst_sizes(ctx, i, C.byref(s)), ctx, "nf_inst_sizes")
+            self.N.append(s.n_interior_cells)
+
+    def __del__(self):
+        if getattr(self, "_ctx", None):
+            A.lib().nf_destroy(self._ctx)
+            self._ctx = None
+
+    def set_state(self, i, U, U_old=None):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.N[i], 5)
+        uo = None if U_old is None else _f64(np.ascontiguousarray(U_old, dtype=np.float64))
+        _check(A.lib().nf_inst_set_state(self._ctx, i, _f64(U), uo), self._ctx, "nf_inst_set_state")
+
+    def set_freestream(self):
+        _check(A.lib().nf_set_freestream(self._ctx), self._ctx, "nf_set_freestream")
+
+    def get_solution(self, i):
+        out = np.zeros((self.N[i], 5))
+        _check(A.lib().nf_inst_get_state(self._ctx, i, _f64(out)), self._ctx, "nf_inst_get_state")
+        return out
+
+    def residuals(self, i, max_n=1 << 20):
+        out = np.zeros(max_n)
+        n = C.c_int32()
+        _check(A.lib().nf_inst_residuals(self._ctx, i, _f64(out), max_n, C.byref(n)), self._ctx,
+               "nf_inst_residuals")
+        return out[: n.value].copy()
+
+    def run(self, n_steps, residual_drop=0.0, check_every=10, use_graph=True):
+        o = A.RunOpts(n_steps, residual_drop, check_every, int(use_graph))
+        st = (A.RunStats * self.n)()
+        _check(A.lib().nf_run_batch(self._ctx, C.byref(o), st), self._ctx, "nf_run_batch")
+        return [{f: getattr(st[i], f) for f, _ in A.RunStats._fields_} for i in range(self.n)]
+
+    def autotune(self, steps=10):
+        return _autotune(self._ctx, steps)
+
+    def set_launch(self, block, xcd_remap=True):
+        _check(A.lib().nf_set_launch(self._ctx, block, int(xcd_remap)), self._ctx, "nf_set_launch")
+
 
 class Driver:
     """The reference's time loop (src/Driver.cpp:31-57) without VTK output."""

@@ -35,6 +35,7 @@ def test_struct_layouts():
     assert C.sizeof(A.Params) == 4 * 4 + 8 + 40 + 4 * 4 + 8  # 5th int + tail padding
     assert C.sizeof(A.MeshDesc) == 8 * 8
     assert C.sizeof(A.RunStats) == 8 + 5 * 8
+    assert C.sizeof(A.TuneResult) == 4 + 4 + 8 + 8 + 8 * 8  # n_tried padded to 8
 
 
 def test_no_gpu_reports_nogpu():
@@ -46,6 +47,20 @@ def test_no_gpu_reports_nogpu():
     p = nf.Config().to_params()
     rc = A.lib().nf_create(C.byref(mesh.desc), C.byref(p), C.byref(ctx))
     assert rc == A.NF_E_NOGPU
+    descs = (A.MeshDesc * 2)(mesh.desc, mesh.desc)
+    params = (A.Params * 2)(p, p)
+    assert A.lib().nf_create_batch(2, descs, params, C.byref(ctx)) == A.NF_E_NOGPU
+
+
+def test_batch_argument_checks():
+    # rejected before any device call: bad count, differing shared options
+    mesh = nf.Mesh.box(2)
+    ctx = C.c_void_p()
+    p = nf.Config().to_params()
+    q = nf.Config(CFL=0.3).to_params()
+    descs = (A.MeshDesc * 2)(mesh.desc, mesh.desc)
+    assert A.lib().nf_create_batch(0, descs, (A.Params * 2)(p, p), C.byref(ctx)) == A.NF_E_ARG
+    assert A.lib().nf_create_batch(2, descs, (A.Params * 2)(p, q), C.byref(ctx)) == A.NF_E_ARG
 
 
 def test_mesh_errors_are_reported():
