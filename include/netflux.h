@@ -151,6 +151,11 @@ typedef struct nf_run_opts {
     double residual_drop;         /* <= 0: run all steps                   */
     int32_t check_every;          /* steps per graph replay (>=1)          */
     int32_t use_graph;            /* 1: hipGraph replay, 0: eager launches */
+    /* stall fallback for warm-started instances (0: off): checked between
+     * replays, an instance whose residual did not fall below stall_ratio
+     * times its value stall_window steps earlier restarts from freestream */
+    int32_t stall_window;
+    double stall_ratio;
 } nf_run_opts;
 typedef struct nf_run_stats {
     int32_t steps_done;
@@ -159,6 +164,8 @@ typedef struct nf_run_stats {
     double last_dt;
     double res_first, res_last;   /* r_1 and r_k                           */
     double elapsed_ms;            /* hipEvent time of the loop             */
+    int32_t fell_back;            /* warm start replaced by the freestream */
+    int32_t pad;
 } nf_run_stats;
 int nf_run(nf_ctx *ctx, const nf_run_opts *opts, nf_run_stats *stats);
 /* Residual history r_k of the last nf_run / nf_step calls (up to max). */
@@ -209,6 +216,12 @@ int nf_inst_residuals(nf_ctx *ctx, int32_t inst, double *out, int32_t max, int32
  * are done; stats[n] per instance, elapsed_ms is the batch wall time. */
 int nf_run_batch(nf_ctx *ctx, const nf_run_opts *opts, nf_run_stats *stats);
 
+/* The drop of instance `inst` is measured against `ref` instead of its first
+ * step's residual (ref <= 0 restores that); kept across state changes. */
+int nf_inst_set_residual_ref(nf_ctx *ctx, int32_t inst, double ref);
+/* ||R_rho||_2 of every instance's current state (no update); out[n] */
+int nf_eval_residuals(nf_ctx *ctx, double *out);
+
 /* Autotuner: times `steps` steps for each launch configuration (threads per
  * block 128 or 256, XCD-contiguous block order on or off) from a copy of the
  * current state, restores the state and keeps the fastest configuration. */
@@ -221,6 +234,57 @@ typedef struct nf_tune_result {
 } nf_tune_result;
 int nf_autotune(nf_ctx *ctx, int32_t steps, nf_tune_result *out);
 int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap);
+
+/* ---- learned warm start ---------------------------------------------------
+ * MLP 6 -> 128 -> 128 -> 128 -> 5 with GELU: per-cell features (Mach, bump
+ * height, centroid x, y, z, wall distance) -> primitive variables
+ * (rho, u, v, w, p).  BF16 weights and activations as MFMA operands, FP32
+ * accumulation, FP32 master weights and Adam state.  No counterpart in the
+ * reference, whose only initial condition is the freestream
+ * (src/SolverData.cpp:27-34). */
+typedef struct nf_mlp nf_mlp;
+/* FP32 parameters, flattened W1 [128][6], b1, W2 [128][128], b2, W3, b3,
+ * W4 [5][128], b4 (each W row-major [out][in]) */
+int64_t nf_mlp_n_params(void);
+int nf_mlp_create(uint64_t seed, nf_mlp **out);
+int nf_mlp_destroy(nf_mlp *m);
+const char *nf_mlp_last_error(const nf_mlp *m);
+int nf_mlp_get_params(nf_mlp *m, float *params);
+int nf_mlp_set_params(nf_mlp *m, const float *params);   /* resets Adam state */
+/* normalisation: x_n = (x - in_shift) * in_scale; y = y_n * out_scale + out_shift */
+int nf_mlp_set_norm(nf_mlp *m, const float *in_shift, const float *in_scale, const float *out_shift,
+                    const float *out_scale);
+int nf_mlp_get_norm(nf_mlp *m, float *in_shift, float *in_scale, float *out_shift, float *out_scale);
+/* raw features x [n][6] -> raw outputs y [n][5] (host arrays) */
+int nf_mlp_predict(nf_mlp *m, const float *x, int64_t n, float *y);
+/* mean squared error of the normalised outputs over one batch of n <= 2^20
+ * samples and its gradient with respect to the FP32 parameters */
+int nf_mlp_loss_grad(nf_mlp *m, const float *x, const float *y, int64_t n, float *loss, float *grad);
+typedef struct nf_train_opts {
+    int32_t epochs;
+    int32_t batch;                /* multiple of 256                        */
+    float lr, beta1, beta2, eps;  /* Adam                                   */
+    float lr_decay;               /* per-epoch factor (0 or 1: constant)    */
+    uint64_t seed;                /* minibatch shuffling                    */
+} nf_train_opts;
+typedef struct nf_train_stats {
+    float loss_first, loss_last;  /* first minibatch, last epoch's last one */
+    int64_t steps;
+    double elapsed_ms;
+} nf_train_stats;
+/* Adam over shuffled minibatches of the n samples x [n][6], y [n][5]; the
+ * normalisation, unless set, becomes the data's mean and standard deviation */
+int nf_mlp_train(nf_mlp *m, const float *x, const float *y, int64_t n, const nf_train_opts *opts,
+                 nf_train_stats *stats);
+/* Network features of instance `inst`: (case_params[0], case_params[1],
+ * centroid x, y, z, distance to the nearest wall-face centroid), x [N][6] in
+ * the instance's reference cell order (wall = slip or no-slip patch). */
+int nf_inst_features(nf_ctx *ctx, int32_t inst, const double *case_params, float *x);
+/* Warm start of instance `inst`: the network's primitive-variable prediction
+ * from the features above becomes U (U_old = U).  If any cell has rho <= 0,
+ * p <= 0 or a non-finite value, the instance starts from the freestream
+ * instead and *fell_back = 1. */
+int nf_inst_warm_start(nf_ctx *ctx, int32_t inst, nf_mlp *net, const double *case_params, int32_t *fell_back);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ NF_FIELD_PRIM, NF_FIELD_LIMITER, NF_FIELD_GRADIENT, NF_FIELD_RESIDUAL, NF_FIELD_
 u32p = C.POINTER(C.c_uint32)
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
 
 
 class MeshDesc(C.Structure):
@@ -54,14 +55,15 @@ class FVSizes(C.Structure):
 
 class RunOpts(C.Structure):
     _fields_ = [("n_steps", C.c_int32), ("residual_drop", C.c_double),
-                ("check_every", C.c_int32), ("use_graph", C.c_int32)]
+                ("check_every", C.c_int32), ("use_graph", C.c_int32),
+                ("stall_window", C.c_int32), ("stall_ratio", C.c_double)]
 
 
 class RunStats(C.Structure):
     _fields_ = [("steps_done", C.c_int32), ("converged", C.c_int32),
                 ("time", C.c_double), ("last_dt", C.c_double),
                 ("res_first", C.c_double), ("res_last", C.c_double),
-                ("elapsed_ms", C.c_double)]
+                ("elapsed_ms", C.c_double), ("fell_back", C.c_int32), ("pad", C.c_int32)]
 
 
 class KernelTime(C.Structure):
@@ -71,6 +73,16 @@ class KernelTime(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("block", C.c_int32), ("xcd_remap", C.c_int32), ("ms_per_step", C.c_double),
                 ("n_tried", C.c_int32), ("tried_ms", C.c_double * 8)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("lr_decay", C.c_float), ("seed", C.c_uint64)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("loss_first", C.c_float), ("loss_last", C.c_float), ("steps", C.c_int64),
+                ("elapsed_ms", C.c_double)]
 
 
 # (name, restype, argtypes) of every symbol declared in include/netflux.h
@@ -107,6 +119,22 @@ SIGNATURES = [
     ("nf_run_batch", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     ("nf_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(TuneResult)]),
     ("nf_set_launch", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("nf_inst_set_residual_ref", C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    ("nf_eval_residuals", C.c_int, [C.c_void_p, f64p]),
+    ("nf_inst_features", C.c_int, [C.c_void_p, C.c_int32, f64p, f32p]),
+    ("nf_inst_warm_start", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, f64p, i32p]),
+    ("nf_mlp_n_params", C.c_int64, []),
+    ("nf_mlp_create", C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("nf_mlp_destroy", C.c_int, [C.c_void_p]),
+    ("nf_mlp_last_error", C.c_char_p, [C.c_void_p]),
+    ("nf_mlp_get_params", C.c_int, [C.c_void_p, f32p]),
+    ("nf_mlp_set_params", C.c_int, [C.c_void_p, f32p]),
+    ("nf_mlp_set_norm", C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p]),
+    ("nf_mlp_get_norm", C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p]),
+    ("nf_mlp_predict", C.c_int, [C.c_void_p, f32p, C.c_int64, f32p]),
+    ("nf_mlp_loss_grad", C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, f32p, f32p]),
+    ("nf_mlp_train", C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, C.POINTER(TrainOpts),
+                               C.POINTER(TrainStats)]),
 ]
 
 _lib = None

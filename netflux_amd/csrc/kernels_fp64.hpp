@@ -52,6 +52,7 @@ __device__ __forceinline__ size_t fix(uint32_t c, uint32_t s, int k) {  // [tile
 struct InstState {
     double dt;         // time step of the current step
     double time;       // sum of completed steps' dt
+    double rref;       // residual the drop is measured against (0: the first step's)
     int nstep;         // completed steps
     int err;           // NF_E_INVALID_DT once raised
     int stop;          // nonzero: kernels skip this instance

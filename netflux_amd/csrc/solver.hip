@@ -20,6 +20,7 @@
 #include "kernels_fp64.hpp"
 #include "kernels_mixed.hpp"
 #include "mesh.hpp"
+#include "mlp.hpp"
 
 using namespace nf;
 
@@ -76,21 +77,28 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_dt(const double *red, const 
 }
 
 // r_k = sqrt(sum of the instance's partials), summed in a fixed order, into
-// the instance's residual history; sets `converged` once r_k / r_0 <= drop.
+// the instance's residual history; sets `converged` once r_k / ref <= drop,
+// ref = rref if set, else r_0.  With `out`, r goes to out[i] instead.
 __global__ __launch_bounds__(BLOCK) void k_finalize_res(const double *red, const uint32_t *c0, const uint32_t *nc,
-                                                        InstState *st, double *hist, int cap, double drop) {
+                                                        InstState *st, double *hist, int cap, double drop,
+                                                        double *out) {
     __shared__ double sh[BLOCK / 64];
     const uint32_t i = blockIdx.x;
-    if (st[i].stop) return;
+    if (!out && st[i].stop) return;
     double v = 0.0;
     for (uint32_t q = threadIdx.x; q < nc[i]; q += BLOCK) v += red[c0[i] + q];
     v = block_reduce<false, BLOCK>(v, sh);
     if (threadIdx.x == 0) {
-        const int k = st[i].nstep;
         const double r = sqrt(v);
+        if (out) {
+            out[i] = r;
+            return;
+        }
+        const int k = st[i].nstep;
         double *h = hist + (size_t)i * cap;
         if (k < cap) h[k] = r;
-        if (drop > 0.0 && k < cap && h[0] > 0.0 && r / h[0] <= drop) st[i].converged = 1;
+        const double ref = st[i].rref > 0.0 ? st[i].rref : h[0];
+        if (drop > 0.0 && k < cap && ref > 0.0 && r / ref <= drop) st[i].converged = 1;
     }
 }
 
@@ -111,8 +119,65 @@ __global__ void k_flags(InstState *st, int n, int op, int which, double dt) {
 }
 
 // fresh run state of instance i (nstep = 0, time = 0)
-__global__ void k_reset_inst(InstState *st, int i, int uold_zero) {
-    if (threadIdx.x == 0) st[i] = InstState{0.0, 0.0, 0, 0, 0, 0, 0, 0, uold_zero, 0};
+__global__ void k_reset_inst(InstState *st, int i, int uold_zero, double rref) {
+    if (threadIdx.x == 0) st[i] = InstState{0.0, 0.0, rref, 0, 0, 0, 0, 0, 0, uold_zero, 0};
+}
+
+// Network features of the cells [0, n) of one instance, feature-major
+// x[6][n]: (p0, p1, centroid, distance to the nearest wall point).  The wall
+// points are streamed through LDS; every thread takes part in every pass.
+__global__ __launch_bounds__(BLOCK) void k_features(DevMesh m, uint32_t cell0, uint32_t n, const float4 *wall,
+                                                    uint32_t nw, float p0, float p1, float *x) {
+    __shared__ float4 sh[BLOCK];
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = c < n;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    if (live) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(m.cnodes + 4 * (size_t)(cell0 + c));
+        const uint32_t id[4] = {q.x, q.y, q.z, q.w};
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) s[d] += m.nodes[4 * (size_t)id[k] + d];
+        cx = (float)(0.25 * s[0]);
+        cy = (float)(0.25 * s[1]);
+        cz = (float)(0.25 * s[2]);
+    }
+    float d2 = INFINITY;
+    for (uint32_t w0 = 0; w0 < nw; w0 += BLOCK) {
+        if (w0 + threadIdx.x < nw) sh[threadIdx.x] = wall[w0 + threadIdx.x];
+        __syncthreads();
+        const uint32_t cnt = min((uint32_t)BLOCK, nw - w0);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float dx = sh[k].x - cx, dy = sh[k].y - cy, dz = sh[k].z - cz;
+            d2 = fminf(d2, dx * dx + dy * dy + dz * dz);
+        }
+        __syncthreads();
+    }
+    if (live) {
+        x[c] = p0;
+        x[n + c] = p1;
+        x[2 * (size_t)n + c] = cx;
+        x[3 * (size_t)n + c] = cy;
+        x[4 * (size_t)n + c] = cz;
+        x[5 * (size_t)n + c] = nw ? sqrtf(d2) : 0.f;
+    }
+}
+
+// predicted primitives V[5][n] -> conservative state of the cells [0, n) of
+// one instance (tiles starting at its first cell); cells with rho <= 0,
+// p <= 0 or a non-finite value are counted in bad_state
+__global__ __launch_bounds__(BLOCK) void k_warm_state(const float *Vp, uint32_t n, double *U, InstState *st, int inst) {
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= n) return;
+    double V[5], Uc[5];
+    for (int k = 0; k < 5; ++k) V[k] = (double)Vp[(size_t)k * n + c];
+    prim_to_cons(V, Uc);
+    bool ok = V[0] > 0.0 && V[4] > 0.0;
+    for (int k = 0; k < 5; ++k) {
+        ok = ok && (Uc[k] - Uc[k] == 0.0);
+        U[tix(c, k)] = Uc[k];
+    }
+    if (!ok) atomicAdd(&st[inst].bad_state, 1);
 }
 
 struct Inst {
@@ -121,6 +186,9 @@ struct Inst {
     std::vector<uint32_t> perm;  // local device cell -> reference cell
     double Vinf[5] = {0, 0, 0, 0, 0};
     bool dt_valid = false;
+    uint32_t wall0 = 0, nwall = 0;  // wall-face centroids of the instance
+    double rref = 0.0;              // residual reference of the drop (0: r_0)
+    bool warm = false;              // state from the network, not yet fallen back
 };
 
 // reduction chunks of whole instances for one sweep block size
@@ -164,6 +232,10 @@ struct nf_ctx {
     double *red_dt = nullptr, *red_res = nullptr;  // per chunk
     double *hist = nullptr;                         // [instances][hist_cap]
     int hist_cap = 0;
+    float4 *wall = nullptr;                         // wall-face centroids of all instances
+    float *feat = nullptr, *vpred = nullptr;        // network features / prediction scratch
+    uint32_t feat_cap = 0;
+    double *r_eval = nullptr;                       // [instances]
     bool aux_valid = false;
     std::vector<void *> owned;
     hipGraphExec_t graph = nullptr;
@@ -331,7 +403,7 @@ int reduce_dt(nf_ctx *ctx, int advance) {
                   (const uint32_t *)r.inst_c0, (const uint32_t *)r.inst_nc, ctx->st, advance);
 }
 
-int reduce_res(nf_ctx *ctx, double drop) {
+int reduce_res(nf_ctx *ctx, double drop, double *out = nullptr) {
     int rc;
     const RedTable &r = red(ctx);
     if ((rc = LAUNCH("reduce_res", k_reduce<false>, dim3(r.nchunk), dim3(BLOCK), (const double *)ctx->res_part,
@@ -339,7 +411,7 @@ int reduce_res(nf_ctx *ctx, double drop) {
         return rc;
     return LAUNCH("finalize_res", k_finalize_res, dim3(ctx->n_inst), dim3(BLOCK), (const double *)ctx->red_res,
                   (const uint32_t *)r.inst_c0, (const uint32_t *)r.inst_nc, ctx->st, ctx->hist, ctx->hist_cap,
-                  drop);
+                  drop, out);
 }
 
 // One step of every live instance (Solver::step, src/Solver.cpp:34-52).  The
@@ -480,11 +552,79 @@ int after_state_set(nf_ctx *ctx, int i, bool uold_zero) {
                            ctx->V + (size_t)5 * I.cell0, I.Npad);
         HIPC(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_reset_inst, dim3(1), dim3(64), 0, ctx->stream, ctx->st, i, (int)uold_zero);
+    hipLaunchKernelGGL(k_reset_inst, dim3(1), dim3(64), 0, ctx->stream, ctx->st, i, (int)uold_zero, I.rref);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(ctx->stream));
     I.dt_valid = false;
+    I.warm = false;
     ctx->aux_valid = false;
+    return NF_OK;
+}
+
+// V = prim(U) over every device cell
+int prim_all(nf_ctx *ctx) {
+    if (ctx->mixed) {
+        HIPC(mx::launch_prim_mx(dim3(ctx->Ncell / BLOCK), ctx->stream, ctx->Ua, ctx->V32, ctx->Ncell));
+    } else {
+        hipLaunchKernelGGL(k_prim, dim3(ctx->Ncell / BLOCK), dim3(BLOCK), 0, ctx->stream, (const double *)ctx->Ua,
+                           ctx->V, ctx->Ncell);
+        HIPC(hipGetLastError());
+    }
+    return NF_OK;
+}
+
+// ||R_rho||_2 of every instance's current state into out[n] (host); the
+// update of the stage-1 sweep goes to Ub and is discarded, V = prim(U) after
+int eval_residuals(nf_ctx *ctx, double *out) {
+    int rc;
+    if ((rc = clear_flags(ctx))) return rc;
+    if ((rc = prim_all(ctx))) return rc;
+    if ((rc = run_sweepA(ctx, false, false))) return rc;
+    if ((rc = run_sweepB(ctx, 1, ctx->Ua, ctx->Ub, true, false))) return rc;
+    if ((rc = reduce_res(ctx, 0.0, ctx->r_eval))) return rc;
+    if ((rc = prim_all(ctx))) return rc;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(out, ctx->r_eval, ctx->n_inst * sizeof(double), hipMemcpyDeviceToHost));
+    for (Inst &I : ctx->inst) I.dt_valid = false;
+    ctx->aux_valid = false;
+    return NF_OK;
+}
+
+// dt from the current primvars for the instances with need[i]; the others
+// keep their device dt (the one their last step computed) untouched
+int timestep_subset(nf_ctx *ctx, const std::vector<bool> &need) {
+    int rc;
+    if (std::all_of(need.begin(), need.end(), [](bool b) { return b; })) return enqueue_timestep(ctx);
+    std::vector<InstState> h;
+    if ((rc = read_states(ctx, h))) return rc;
+    std::vector<InstState> masked = h;
+    for (int i = 0; i < ctx->n_inst; ++i)
+        if (!need[i]) masked[i].stop = 1;
+    HIPC(hipMemcpy(ctx->st, masked.data(), h.size() * sizeof(InstState), hipMemcpyHostToDevice));
+    if ((rc = enqueue_timestep(ctx))) return rc;
+    std::vector<InstState> after;
+    if ((rc = read_states(ctx, after))) return rc;
+    for (int i = 0; i < ctx->n_inst; ++i)
+        if (!need[i]) after[i] = h[i];
+    HIPC(hipMemcpy(ctx->st, after.data(), h.size() * sizeof(InstState), hipMemcpyHostToDevice));
+    return NF_OK;
+}
+
+// feature-major features of instance i into ctx->feat
+int features(nf_ctx *ctx, int i, const double *cp) {
+    const Inst &I = ctx->inst[i];
+    if (I.N > ctx->feat_cap) {
+        uint32_t cap = 0;
+        for (const Inst &J : ctx->inst) cap = std::max(cap, J.N);
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->feat, 6 * (size_t)cap))) return rc;
+        if ((rc = dalloc(ctx, &ctx->vpred, 5 * (size_t)cap))) return rc;
+        ctx->feat_cap = cap;
+    }
+    hipLaunchKernelGGL(k_features, dim3((I.N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream, devmesh(ctx),
+                       I.cell0, I.N, (const float4 *)(ctx->wall + I.wall0), I.nwall, (float)cp[0], (float)cp[1],
+                       ctx->feat);
+    HIPC(hipGetLastError());
     return NF_OK;
 }
 
@@ -520,6 +660,8 @@ int set_freestream(nf_ctx *ctx, int i) {
     }
     return NF_OK;
 }
+
+double d_node(const double *nodes, uint32_t v, int d) { return nodes[3 * (size_t)v + d]; }
 
 int build_red_table(nf_ctx *ctx, uint32_t B, RedTable &r) {
     std::vector<uint32_t> first, n, c0, nc;
@@ -572,6 +714,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
 
     // host pipeline per instance; the device arrays are the concatenation
     std::vector<uint32_t> cnodes, nbr, fcode, bface, inst_end;
+    std::vector<float4> wall;
     std::vector<uint8_t> bbc;
     std::vector<uint16_t> binst, blk_inst;
     std::vector<double> nodes4;
@@ -625,6 +768,18 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
             bbc.push_back(L.bbc[q]);
             binst.push_back((uint16_t)i);
         }
+        // wall-face centroids (slip and no-slip patches) for the network features
+        I.wall0 = (uint32_t)wall.size();
+        for (size_t q = 0; q < g.pfirst.size(); ++q) {
+            if (g.pbc[q] == NF_BC_FARFIELD) continue;
+            for (uint32_t f = g.pfirst[q]; f < g.pfirst[q] + g.pn[q]; ++f) {
+                double cf[3] = {0, 0, 0};
+                for (int k = 0; k < 3; ++k)
+                    for (int e = 0; e < 3; ++e) cf[e] += d_node(d->nodes, g.fnodes[3 * (size_t)f + k], e);
+                wall.push_back(make_float4((float)(cf[0] / 3), (float)(cf[1] / 3), (float)(cf[2] / 3), 0.f));
+            }
+        }
+        I.nwall = (uint32_t)wall.size() - I.wall0;
         for (uint32_t b = 0; b < I.Npad / BLOCK; ++b) blk_inst.push_back((uint16_t)i);
         inst_end.push_back((uint32_t)cell0 + g.N);
         for (uint32_t v = 0; v < d->n_nodes; ++v)
@@ -695,6 +850,9 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     if ((rc = build_red_table(ctx, 128, ctx->red128))) return rc;
     if ((rc = dalloc(ctx, &ctx->red_dt, (size_t)ctx->red128.nchunk))) return rc;
     if ((rc = dalloc(ctx, &ctx->red_res, (size_t)ctx->red128.nchunk))) return rc;
+    if ((rc = dalloc(ctx, &ctx->r_eval, (size_t)n))) return rc;
+    if ((rc = dalloc(ctx, &ctx->wall, wall.size()))) return rc;
+    if (!wall.empty()) HIPC(hipMemcpy(ctx->wall, wall.data(), wall.size() * sizeof(float4), hipMemcpyHostToDevice));
     if ((rc = ensure_hist(ctx, 1024))) return rc;
     for (int i = 0; i < n; ++i)
         if ((rc = set_freestream(ctx, i))) return rc;
@@ -720,15 +878,18 @@ int step_device_dt(nf_ctx *ctx) {
 }
 
 // device-resident loop over all instances, K steps per graph launch; the host
-// reads the stop flags between launches only when a residual drop is set
+// reads the run flags between launches when a residual drop or the stall
+// fallback is set
 int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
     if (!o || o->n_steps < 0) return NF_E_ARG;
     int rc;
     const int K = std::max(1, o->check_every);
+    const int n = ctx->n_inst;
     if ((rc = clear_flags(ctx))) return rc;
-    bool need_dt = false;
-    for (const Inst &I : ctx->inst) need_dt |= !I.dt_valid;
-    if (need_dt && (rc = enqueue_timestep(ctx))) return rc;
+    std::vector<bool> need(n);
+    for (int i = 0; i < n; ++i) need[i] = !ctx->inst[i].dt_valid;
+    if (std::any_of(need.begin(), need.end(), [](bool b) { return b; }) && (rc = timestep_subset(ctx, need)))
+        return rc;
     for (Inst &I : ctx->inst) I.dt_valid = true;
     std::vector<InstState> h0;
     if ((rc = read_states(ctx, h0))) return rc;
@@ -736,6 +897,9 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
     for (auto &s : h0) maxstep = std::max(maxstep, s.nstep);
     if ((rc = ensure_hist(ctx, maxstep + o->n_steps + 2))) return rc;
     const double drop = o->residual_drop;
+    const bool stall = o->stall_window > 0;
+    std::vector<int> base(n), acc(n, 0), fell(n, 0);
+    for (int i = 0; i < n; ++i) base[i] = h0[i].nstep;
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
@@ -770,9 +934,33 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
                 if ((rc = enqueue_step(ctx, drop))) return rc;
         }
         done += chunk;
-        if (drop > 0.0 && done < o->n_steps) {
+        if ((drop > 0.0 || stall) && done < o->n_steps) {
             std::vector<InstState> h;
             if ((rc = read_states(ctx, h))) return rc;
+            if (stall) {
+                std::vector<bool> reset(n, false);
+                for (int i = 0; i < n; ++i) {
+                    const int k = h[i].nstep;
+                    if (!ctx->inst[i].warm || h[i].stop || k <= o->stall_window || k > ctx->hist_cap) continue;
+                    double r_now = 0, r_then = 0;
+                    const double *hp = ctx->hist + (size_t)i * ctx->hist_cap;
+                    HIPC(hipMemcpy(&r_now, hp + k - 1, sizeof(double), hipMemcpyDeviceToHost));
+                    HIPC(hipMemcpy(&r_then, hp + k - 1 - o->stall_window, sizeof(double), hipMemcpyDeviceToHost));
+                    if (!(r_now <= o->stall_ratio * r_then)) reset[i] = true;
+                }
+                if (std::any_of(reset.begin(), reset.end(), [](bool b) { return b; })) {
+                    for (int i = 0; i < n; ++i) {
+                        if (!reset[i]) continue;
+                        acc[i] += h[i].nstep - base[i];
+                        base[i] = 0;
+                        fell[i] = 1;
+                        if ((rc = set_freestream(ctx, i))) return rc;
+                    }
+                    if ((rc = timestep_subset(ctx, reset))) return rc;
+                    for (Inst &I : ctx->inst) I.dt_valid = true;
+                    if ((rc = read_states(ctx, h))) return rc;
+                }
+            }
             stopped = true;
             for (auto &s : h) stopped &= s.stop != 0;
         }
@@ -787,13 +975,15 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
     std::vector<InstState> h;
     if ((rc = read_states(ctx, h))) return rc;
     if (stats) {
-        for (int i = 0; i < ctx->n_inst; ++i) {
+        for (int i = 0; i < n; ++i) {
             nf_run_stats &s = stats[i];
-            s.steps_done = h[i].nstep - h0[i].nstep;
+            s.steps_done = acc[i] + h[i].nstep - base[i];
             s.converged = h[i].converged;
             s.time = h[i].time;
             s.last_dt = h[i].dt;
             s.elapsed_ms = ms;
+            s.fell_back = fell[i];
+            s.pad = 0;
             double r[2] = {0, 0};
             const double *hp = ctx->hist + (size_t)i * ctx->hist_cap;
             HIPC(hipMemcpy(&r[0], hp, sizeof(double), hipMemcpyDeviceToHost));
@@ -983,16 +1173,13 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
         if ((rc = dalloc(ctx, &ctx->R_aux, 5 * NC))) return rc;
     }
     if ((rc = clear_flags(ctx))) return rc;
-    if (ctx->mixed) {
-        HIPC(mx::launch_prim_mx(dim3(ctx->Ncell / BLOCK), ctx->stream, ctx->Ua, ctx->V32, ctx->Ncell));
-    } else {
-        hipLaunchKernelGGL(k_prim, dim3(ctx->Ncell / BLOCK), dim3(BLOCK), 0, ctx->stream, (const double *)ctx->Ua,
-                           ctx->V, ctx->Ncell);
-        HIPC(hipGetLastError());
-    }
+    if ((rc = prim_all(ctx))) return rc;
     if ((rc = run_sweepA(ctx, false, true))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, ctx->Ub, false, true))) return rc;
+    // the sweep wrote prim of the discarded update: primvars = prim(U) again
+    if ((rc = prim_all(ctx))) return rc;
     ctx->aux_valid = true;
+    ctx->inst[0].dt_valid = false;
     return check_errors(ctx);
 }
 
@@ -1049,6 +1236,56 @@ int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out, int32_t m
         ++n;
     }
     if (n_out) *n_out = n;
+    return NF_OK;
+}
+
+int nf_inst_set_residual_ref(nf_ctx *ctx, int32_t i, double ref) {
+    if (!valid_inst(ctx, i)) return NF_E_ARG;
+    ctx->inst[i].rref = ref > 0.0 ? ref : 0.0;
+    std::vector<InstState> h;
+    int rc;
+    if ((rc = read_states(ctx, h))) return rc;
+    h[i].rref = ctx->inst[i].rref;
+    HIPC(hipMemcpy(ctx->st + i, &h[i], sizeof(InstState), hipMemcpyHostToDevice));
+    return NF_OK;
+}
+
+int nf_eval_residuals(nf_ctx *ctx, double *out) {
+    if (!ctx || !out) return NF_E_ARG;
+    return eval_residuals(ctx, out);
+}
+
+int nf_inst_features(nf_ctx *ctx, int32_t i, const double *cp, float *x) {
+    if (!valid_inst(ctx, i) || !cp || !x) return NF_E_ARG;
+    int rc;
+    if ((rc = features(ctx, i, cp))) return rc;
+    const Inst &I = ctx->inst[i];
+    std::vector<float> fm((size_t)6 * I.N);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemcpy(fm.data(), ctx->feat, fm.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < I.N; ++c)
+        for (int f = 0; f < 6; ++f) x[6 * (size_t)I.perm[c] + f] = fm[(size_t)f * I.N + c];
+    return NF_OK;
+}
+
+int nf_inst_warm_start(nf_ctx *ctx, int32_t i, nf_mlp *net, const double *cp, int32_t *fell_back) {
+    if (!valid_inst(ctx, i) || !net || !cp) return NF_E_ARG;
+    int rc;
+    if ((rc = features(ctx, i, cp))) return rc;
+    Inst &I = ctx->inst[i];
+    HIPC(nf::mlp::predict_device(net, ctx->feat, I.N, ctx->vpred, ctx->stream));
+    if ((rc = clear_flags(ctx))) return rc;
+    hipLaunchKernelGGL(k_warm_state, dim3((I.N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, ctx->stream,
+                       (const float *)ctx->vpred, I.N, ctx->Ua + (size_t)5 * I.cell0, ctx->st, i);
+    HIPC(hipGetLastError());
+    std::vector<InstState> h;
+    if ((rc = read_states(ctx, h))) return rc;
+    const bool bad = h[i].bad_state > 0;
+    if (fell_back) *fell_back = bad ? 1 : 0;
+    if (bad) return set_freestream(ctx, i);
+    // U_old = U: the first RK3 step must not scale a warm start
+    if ((rc = after_state_set(ctx, i, false))) return rc;
+    I.warm = true;
     return NF_OK;
 }
 

@@ -342,8 +342,32 @@ class BatchSolver:
                "nf_inst_residuals")
         return out[: n.value].copy()
 
-    def run(self, n_steps, residual_drop=0.0, check_every=10, use_graph=True):
-        o = A.RunOpts(n_steps, residual_drop, check_every, int(use_graph))
+    def features(self, i, case_params):
+        """Network features [N][6] of instance i (reference cell order)."""
+        cp = np.ascontiguousarray(case_params, np.float64)
+        x = np.zeros((self.N[i], 6), np.float32)
+        _check(A.lib().nf_inst_features(self._ctx, i, _f64(cp), x.ctypes.data_as(A.f32p)), self._ctx,
+               "nf_inst_features")
+        return x
+
+    def warm_start(self, i, net, case_params):
+        """Network prediction as the state of instance i; True if it fell back to the freestream."""
+        cp = np.ascontiguousarray(case_params, np.float64)
+        fb = C.c_int32()
+        _check(A.lib().nf_inst_warm_start(self._ctx, i, net._h, _f64(cp), C.byref(fb)), self._ctx,
+               "nf_inst_warm_start")
+        return bool(fb.value)
+
+    def set_residual_ref(self, i, ref):
+        _check(A.lib().nf_inst_set_residual_ref(self._ctx, i, float(ref)), self._ctx, "nf_inst_set_residual_ref")
+
+    def eval_residuals(self):
+        out = np.zeros(self.n)
+        _check(A.lib().nf_eval_residuals(self._ctx, _f64(out)), self._ctx, "nf_eval_residuals")
+        return out
+
+    def run(self, n_steps, residual_drop=0.0, check_every=10, use_graph=True, stall_window=0, stall_ratio=0.0):
+        o = A.RunOpts(n_steps, residual_drop, check_every, int(use_graph), stall_window, stall_ratio)
         st = (A.RunStats * self.n)()
         _check(A.lib().nf_run_batch(self._ctx, C.byref(o), st), self._ctx, "nf_run_batch")
         return [{f: getattr(st[i], f) for f, _ in A.RunStats._fields_} for i in range(self.n)]
