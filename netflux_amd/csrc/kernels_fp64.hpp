@@ -38,6 +38,13 @@
 
 namespace nf {
 
+#ifndef NF_A_PREFETCH
+#define NF_A_PREFETCH 0
+#endif
+#ifndef NF_B_GROUP
+#define NF_B_GROUP 4
+#endif
+
 constexpr int BLOCK = 256;  // threads of the auxiliary kernels; instance padding unit
 constexpr int TILE = 64;    // cells per layout tile
 
@@ -69,7 +76,7 @@ struct DevMesh {
     const uint32_t *cnodes;
     const uint32_t *nbr;
     const uint32_t *fcode;
-    const uint16_t *blk_inst;  // [blocks] instance of each 256-cell block
+    const uint32_t *blk_inst;  // [blocks] instance of each 256-cell block (32-bit: scalar loads)
     const uint32_t *inst_end;  // [instances] one past the instance's last device cell
     InstState *st;             // [instances]
 };
@@ -254,7 +261,10 @@ struct CellGeo {
     Rcp ivol;
 };
 
-__device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo &g) {
+// Node loads of cell c.  Issued before the state gathers of a sweep: the
+// memory counter retires loads in order, so waiting for the nodes then does
+// not wait for the gathers.
+__device__ __forceinline__ void load_nodes(const DevMesh &m, uint32_t c, CellGeo &g) {
     const uint4 q = *reinterpret_cast<const uint4 *>(m.cnodes + 4 * (size_t)c);
     auto node = [&](uint32_t n) {
         const double2 a = *reinterpret_cast<const double2 *>(m.nodes + 4 * (size_t)n);
@@ -264,11 +274,20 @@ __device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo 
     g.p1 = node(q.y);
     g.p2 = node(q.z);
     g.p3 = node(q.w);
+}
+
+// volume, centroid and 1/volume of loaded nodes
+__device__ __forceinline__ void cell_geo(const DevMesh &m, CellGeo &g) {
     // tetrahedron_calc_volume / _centroid (src/geometry/GeometryUtilities.cpp:289-316)
     const V3d ab = vsub(g.p1, g.p0), ac = vsub(g.p2, g.p0), ad = vsub(g.p3, g.p0);
     g.vol = rdiv(dot3(vcross(ab, ac), ad), rcp_of(m.six));
     g.cen = vscl(1.0 / 4, vadd(vadd(vadd(g.p0, g.p1), g.p2), g.p3));
     g.ivol = rcp_of(g.vol);
+}
+
+__device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo &g) {
+    load_nodes(m, c, g);
+    cell_geo(m, g);
 }
 
 // The four nodes of each lane's cell staged in LDS, [12][B] (node q,
@@ -417,19 +436,28 @@ __global__ __launch_bounds__(B) void k_sweepA(DevMesh m, SweepAArgs a) {
     double Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    CellGeo g;
+    load_nodes(m, c, g);
+    __builtin_amdgcn_sched_barrier(0);
+    // neighbour (or ghost) states; NF_A_PREFETCH = 1 loads variable k + 1
+    // while variable k is processed instead of all five up front, which
+    // trades memory-level parallelism for registers
     double Vn[4][5];
+    const double *nsrc[4];
+    uint32_t nstride[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t idx = w[s] & 0x1FFFFFFFu;
+        nsrc[s] = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
+        nstride[s] = (w[s] & 0x80000000u) ? 1 : TILE;
+    }
     if (ORDER != 0) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const uint32_t idx = w[s] & 0x1FFFFFFFu;
-            const double *src = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
-            const size_t stride = (w[s] & 0x80000000u) ? 1 : TILE;
+        for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k)
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
-        }
+            for (int s = 0; s < 4; ++s) Vn[s][k] = nsrc[s][k * nstride[s]];
     }
-    CellGeo g;
-    load_cell(m, c, g);
+    cell_geo(m, g);
     stage_nodes<B>(lds, g);
     V3d S[4], r[4];
 #pragma unroll
@@ -448,6 +476,10 @@ __global__ __launch_bounds__(B) void k_sweepA(DevMesh m, SweepAArgs a) {
         // Barth factor and the face values depend on variable k only.
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
+            if (NF_A_PREFETCH && k + 1 < 5) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) Vn[s][k + 1] = nsrc[s][(k + 1) * nstride[s]];
+            }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
             double g0 = 0.0, g1 = 0.0, g2 = 0.0;
@@ -526,38 +558,48 @@ __global__ __launch_bounds__(B) void k_sweepB(DevMesh m, SweepBArgs a) {
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    // face values of both sides of the four faces, all loads issued up front
-    double Vo4[4][5], Vn4[4][5];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
-        const bool bnd = w[s] & 0x80000000u;
-        const double *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
-        const size_t stride = bnd ? 1 : TILE;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            Vo4[s][k] = a.FV[fix(c, s, k)];
-            Vn4[s][k] = src[k * stride];
-        }
-    }
+    // face values of both sides, loaded NF_B_GROUP faces at a time (4: all
+    // loads issued up front; 2: half the registers, two load phases)
     CellGeo g;
-    load_cell(m, c, g);
-    stage_nodes<B>(lds, g);
+    load_nodes(m, c, g);
+    __builtin_amdgcn_sched_barrier(0);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const uint32_t byte = (code >> (8 * s)) & 0xFFu;
-        V3d S;
-        face_geo<B>(m, g, lds, byte, S, nullptr);
-        // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
-        double UO[5], UN[5], F[5];
-        prim_to_cons(Vo4[s], UO);
-        prim_to_cons(Vn4[s], UN);
-        const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
-        rusanov_side(UO, UN, S, mneg, F);
+    for (int g0 = 0; g0 < 4; g0 += NF_B_GROUP) {
+        double Vo4[NF_B_GROUP][5], Vn4[NF_B_GROUP][5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
-        __builtin_amdgcn_sched_barrier(0);
+        for (int q = 0; q < NF_B_GROUP; ++q) {
+            const int s = g0 + q;
+            const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
+            const bool bnd = w[s] & 0x80000000u;
+            const double *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
+            const size_t stride = bnd ? 1 : TILE;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                Vo4[q][k] = a.FV[fix(c, s, k)];
+                Vn4[q][k] = src[k * stride];
+            }
+        }
+        if (g0 == 0) {
+            cell_geo(m, g);
+            stage_nodes<B>(lds, g);
+        }
+#pragma unroll
+        for (int q = 0; q < NF_B_GROUP; ++q) {
+            const int s = g0 + q;
+            const uint32_t byte = (code >> (8 * s)) & 0xFFu;
+            V3d S;
+            face_geo<B>(m, g, lds, byte, S, nullptr);
+            // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
+            double UO[5], UN[5], F[5];
+            prim_to_cons(Vo4[q], UO);
+            prim_to_cons(Vn4[q], UN);
+            const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
+            rusanov_side(UO, UN, S, mneg, F);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
     if (live && a.check) {
         bool ok = true;

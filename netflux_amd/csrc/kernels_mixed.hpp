@@ -157,6 +157,9 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     float Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    CellGeo g;
+    load_nodes(m, c, g);  // before the gathers: waiting for nodes must not wait for them
+    __builtin_amdgcn_sched_barrier(0);
     float Vn[4][5];
     if (ORDER != 0) {
 #pragma unroll
@@ -168,8 +171,7 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
             for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
         }
     }
-    CellGeo g;
-    load_cell(m, c, g);
+    cell_geo(m, g);
     stage_nodes<B>(lds, g);
     V3f S[4], r[4];
 #pragma unroll
@@ -243,6 +245,9 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    CellGeo g;
+    load_nodes(m, c, g);
+    __builtin_amdgcn_sched_barrier(0);
     float Vo4[4][5], Vn4[4][5];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -256,8 +261,7 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
             Vn4[s][k] = src[k * stride];
         }
     }
-    CellGeo g;
-    load_cell(m, c, g);
+    cell_geo(m, g);
     stage_nodes<B>(lds, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll

@@ -212,7 +212,7 @@ struct nf_ctx {
     bool mixed = false;
     double *nodes = nullptr;
     uint32_t *cnodes = nullptr, *nbr = nullptr, *fcode = nullptr;
-    uint16_t *blk_inst = nullptr;
+    uint32_t *blk_inst = nullptr;
     uint32_t *inst_end = nullptr;
     InstState *st = nullptr;
     double *Ua = nullptr, *Ub = nullptr, *V = nullptr, *FV = nullptr;
@@ -716,7 +716,8 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     std::vector<uint32_t> cnodes, nbr, fcode, bface, inst_end;
     std::vector<float4> wall;
     std::vector<uint8_t> bbc;
-    std::vector<uint16_t> binst, blk_inst;
+    std::vector<uint16_t> binst;
+    std::vector<uint32_t> blk_inst;
     std::vector<double> nodes4;
     ctx->inst.resize(n);
     uint64_t cell0 = 0, node0 = 0;
@@ -780,7 +781,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
             }
         }
         I.nwall = (uint32_t)wall.size() - I.wall0;
-        for (uint32_t b = 0; b < I.Npad / BLOCK; ++b) blk_inst.push_back((uint16_t)i);
+        for (uint32_t b = 0; b < I.Npad / BLOCK; ++b) blk_inst.push_back((uint32_t)i);
         inst_end.push_back((uint32_t)cell0 + g.N);
         for (uint32_t v = 0; v < d->n_nodes; ++v)
             nodes4.insert(nodes4.end(),
@@ -829,7 +830,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     HIPC(hipMemcpy(ctx->cnodes, cnodes.data(), cnodes.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->fcode, fcode.data(), fcode.size() * 4, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(ctx->blk_inst, blk_inst.data(), blk_inst.size() * 2, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->blk_inst, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->inst_end, inst_end.data(), inst_end.size() * 4, hipMemcpyHostToDevice));
     if (ctx->nbf) {
         HIPC(hipMemcpy(ctx->bface, bface.data(), bface.size() * 4, hipMemcpyHostToDevice));
