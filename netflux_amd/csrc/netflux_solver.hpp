@@ -155,6 +155,54 @@ inline VecField SolverData::get_primvars_gradient() const { return fetch(s_, NF_
 inline VecField SolverData::get_primvars_limiter() const { return fetch(s_, NF_FIELD_LIMITER, 5); }
 inline VecField SolverData::get_flux_balance() const { return fetch(s_, NF_FIELD_RESIDUAL, 5); }
 
+// Several meshes advanced together, one launch per sweep over all of them
+// (no reference counterpart: the reference asserts a single solver,
+// src/Driver.cpp:36).  Only the freestream may differ between the configs.
+class BatchSolver {
+    nf_ctx *ctx_ = nullptr;
+    std::vector<uint32_t> n_;
+
+  public:
+    BatchSolver(const std::vector<Config> &configs, const std::vector<nf_mesh_desc> &meshes) {
+        if (configs.size() != meshes.size() || meshes.empty()) throw std::invalid_argument("one config per mesh");
+        std::vector<nf_params> p;
+        for (const Config &c : configs) p.push_back(c.to_params());
+        check(nf_create_batch((int32_t)meshes.size(), meshes.data(), p.data(), &ctx_), nullptr, "nf_create_batch");
+        for (size_t i = 0; i < meshes.size(); ++i) {
+            nf_fv_sizes s{};
+            check(nf_inst_sizes(ctx_, (int32_t)i, &s), ctx_, "nf_inst_sizes");
+            n_.push_back(s.n_interior_cells);
+        }
+    }
+    ~BatchSolver() { nf_destroy(ctx_); }
+    BatchSolver(const BatchSolver &) = delete;
+    BatchSolver &operator=(const BatchSolver &) = delete;
+
+    size_t size() const { return n_.size(); }
+    uint32_t n_cells(size_t i) const { return n_[i]; }
+    void set_state(size_t i, const std::vector<double> &U, const std::vector<double> *U_old = nullptr) {
+        check(nf_inst_set_state(ctx_, (int32_t)i, U.data(), U_old ? U_old->data() : nullptr), ctx_,
+              "nf_inst_set_state");
+    }
+    std::vector<double> get_solution(size_t i) const {
+        std::vector<double> U((size_t)5 * n_[i]);
+        check(nf_inst_get_state(ctx_, (int32_t)i, U.data()), ctx_, "nf_inst_get_state");
+        return U;
+    }
+    std::vector<nf_run_stats> run(int n_steps, double residual_drop = 0.0, int check_every = 10) {
+        nf_run_opts o{n_steps, residual_drop, check_every, 1, 0, 0.0};
+        std::vector<nf_run_stats> st(n_.size());
+        check(nf_run_batch(ctx_, &o, st.data()), ctx_, "nf_run_batch");
+        return st;
+    }
+    nf_tune_result autotune(int steps = 10) {
+        nf_tune_result r{};
+        check(nf_autotune(ctx_, steps, &r), ctx_, "nf_autotune");
+        return r;
+    }
+    nf_ctx *handle() const { return ctx_; }
+};
+
 // src/Driver.cpp:31-57 without output
 class Driver {
     Config &config_;

@@ -160,3 +160,100 @@ def train_network(x, y, epochs=30, batch=4096, lr=2e-3, lr_decay=0.92, seed=0):
     net = Mlp(seed)
     stats = net.train(x, y, epochs=epochs, batch=batch, lr=lr, lr_decay=lr_decay, seed=seed)
     return net, stats
+
+
+def _log(msg):
+    import sys
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _summary(stats):
+    steps = [s["steps_done"] for s in stats]
+    return {"converged": int(sum(s["converged"] for s in stats)), "n": len(stats),
+            "steps_min": int(min(steps)), "steps_median": float(np.median(steps)), "steps_max": int(max(steps)),
+            "fell_back": int(sum(s.get("fell_back", 0) for s in stats))}
+
+
+def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60000, check_every=200,
+          train_cases=0, train_cells=2e5, per_case=4000, epochs=30, train_drop=1e-6, plain=True,
+          plain_lts=False):
+    """Plain FP64 solves one by one (the reference's global time step unless
+    plain_lts), the batched mixed-precision solve with local time steps, and
+    (train_cases > 0) the same batch from a learned warm start trained on
+    train_cases plain solutions of train_cells cells."""
+    cases = sample_cases(n_cases, seed, cells_lo, cells_hi)
+    rep = {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "plain_lts": plain_lts}
+    plain_U, plain_wall, plain_stats = [], [], []
+    if plain:
+        for i, c in enumerate(cases):
+            run = BatchRun([c], precision="fp64", local_time_step=plain_lts)
+            st, wall = run.solve(drop, max_steps, check_every)
+            plain_stats.append(st[0])
+            plain_wall.append(wall)
+            plain_U.append(run.solutions()[0])
+            _log(f"[plain] case {i} cells {c.n_cells} steps {st[0]['steps_done']} conv {st[0]['converged']} "
+                 f"{wall:.2f} s")
+            del run
+        rep["plain_fp64"] = _summary(plain_stats) | {"wall_total_s": sum(plain_wall),
+                                                       "per_instance_s": sum(plain_wall) / n_cases}
+
+    def compare(sols):
+        out = []
+        for i, (c, U) in enumerate(zip(cases, sols)):
+            if not plain_U:
+                break
+            m = mesh_of(c)
+            _, p = wall_pressure(m, U)
+            _, p0 = wall_pressure(m, plain_U[i])
+            out.append(float(np.max(np.abs(p - p0))))
+        return {"wall_p_maxabs_max": max(out), "wall_p_maxabs_median": float(np.median(out))} if out else {}
+
+    run = BatchRun(cases, precision="mixed", autotune_steps=20)
+    st, wall = run.solve(drop, max_steps, check_every)
+    _log(f"[batch mixed] {_summary(st)} {wall:.2f} s, tune {run.tune}")
+    rep["batch_mixed"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases, "tune": run.tune} | \
+        compare(run.solutions())
+    if train_cases:
+        tcases = sample_cases(train_cases, seed + 1000, train_cells, train_cells)
+        t0 = time.perf_counter()
+        trun = BatchRun(tcases, precision="fp64")
+        tst, twall = trun.solve(train_drop, max_steps, check_every)
+        x, y = training_set(trun, per_case, seed)
+        del trun
+        _log(f"[train data] {_summary(tst)} {twall:.2f} s, {len(x)} samples")
+        net, tstats = train_network(x, y, epochs=epochs)
+        _log(f"[train] {tstats}")
+        rep["training"] = {"solves": _summary(tst), "solve_wall_s": twall, "samples": int(len(x)),
+                           "train": tstats, "total_s": time.perf_counter() - t0}
+        st, wall = run.solve(drop, max_steps, check_every, net=net, stall_window=4000, stall_ratio=0.5)
+        _log(f"[batch mixed warm] {_summary(st)} {wall:.2f} s")
+        rep["batch_mixed_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases} | \
+            compare(run.solutions())
+    return rep
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="steady bump-channel study")
+    ap.add_argument("--cases", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cells-lo", type=float, default=2e4)
+    ap.add_argument("--cells-hi", type=float, default=3e5)
+    ap.add_argument("--drop", type=float, default=1e-8)
+    ap.add_argument("--max-steps", type=int, default=60000)
+    ap.add_argument("--check-every", type=int, default=200)
+    ap.add_argument("--train-cases", type=int, default=0)
+    ap.add_argument("--train-cells", type=float, default=2e5)
+    ap.add_argument("--per-case", type=int, default=4000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-lts", action="store_true")
+    a = ap.parse_args(argv)
+    rep = study(a.cases, a.seed, a.cells_lo, a.cells_hi, a.drop, a.max_steps, a.check_every, a.train_cases,
+                a.train_cells, a.per_case, a.epochs, plain=not a.no_plain, plain_lts=a.plain_lts)
+    print(json.dumps(rep))
+
+
+if __name__ == "__main__":
+    main()
