@@ -44,6 +44,18 @@ namespace nf {
 #ifndef NF_B_GROUP
 #define NF_B_GROUP 4
 #endif
+#ifndef NF_B_NODES_FIRST
+#define NF_B_NODES_FIRST 1
+#endif
+// waves per SIMD the sweeps are compiled for (0: the compiler's choice)
+#ifndef NF_SWEEP_WAVES
+#define NF_SWEEP_WAVES 0
+#endif
+#if NF_SWEEP_WAVES
+#define NF_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(NF_SWEEP_WAVES, NF_SWEEP_WAVES)))
+#else
+#define NF_SWEEP_ATTR
+#endif
 
 constexpr int BLOCK = 256;  // threads of the auxiliary kernels; instance padding unit
 constexpr int TILE = 64;    // cells per layout tile
@@ -421,7 +433,7 @@ struct SweepAArgs {
 
 // ORDER: 0 first order, 1 second order without limiter, 2 second order Barth
 template <int ORDER, int B>
-__global__ __launch_bounds__(B) void k_sweepA(DevMesh m, SweepAArgs a) {
+__global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
@@ -546,7 +558,7 @@ struct SweepBArgs {
 };
 
 template <int B>
-__global__ __launch_bounds__(B) void k_sweepB(DevMesh m, SweepBArgs a) {
+__global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
@@ -561,8 +573,10 @@ __global__ __launch_bounds__(B) void k_sweepB(DevMesh m, SweepBArgs a) {
     // face values of both sides, loaded NF_B_GROUP faces at a time (4: all
     // loads issued up front; 2: half the registers, two load phases)
     CellGeo g;
-    load_nodes(m, c, g);
-    __builtin_amdgcn_sched_barrier(0);
+    if (NF_B_NODES_FIRST) {
+        load_nodes(m, c, g);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int g0 = 0; g0 < 4; g0 += NF_B_GROUP) {
@@ -581,6 +595,7 @@ __global__ __launch_bounds__(B) void k_sweepB(DevMesh m, SweepBArgs a) {
             }
         }
         if (g0 == 0) {
+            if (!NF_B_NODES_FIRST) load_nodes(m, c, g);
             cell_geo(m, g);
             stage_nodes<B>(lds, g);
         }

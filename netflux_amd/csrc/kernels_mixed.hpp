@@ -87,6 +87,12 @@ __device__ __forceinline__ void prim_to_cons_f(const float *V, float *U) {
 // Rusanov flux seen from one cell (see rusanov_side): F for UL = own if the
 // cell is the owner (mneg = 0), UL = neighbour otherwise
 __device__ __forceinline__ void rusanov_side_f(const float *UO, const float *UN, V3f S, uint32_t mneg, float *F) {
+    // No contraction here: the two cells of a face evaluate this with UO and
+    // UN swapped, and a fused FO + FN would round the owner's product on one
+    // side and the neighbour's on the other.  Unfused, every operation is
+    // either symmetric in (UO, UN) or exactly negated, so both cells get the
+    // same flux bit for bit and interior faces conserve exactly.
+#pragma clang fp contract(off)
     const float z = S.x * S.x + S.y * S.y + S.z * S.z;
     const float area = fsqrt(z);
     const float ia = frcp(area);

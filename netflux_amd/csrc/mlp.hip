@@ -67,10 +67,26 @@ struct Gemm {
 __device__ __forceinline__ void load_tile(const __bf16 *P, int64_t s_row, int64_t s_k, int r0, int k0, int R,
                                           int kend, __bf16 (*S)[LDK]) {
     const int t = threadIdx.x;
-    const bool vec = s_k == 1 && (s_row & 7) == 0 && (((uintptr_t)P) & 15) == 0;
+    const bool aligned = (((uintptr_t)P) & 15) == 0;
+    const bool vec = s_k == 1 && (s_row & 7) == 0 && aligned;
+    const bool vec_rows = !vec && s_row == 1 && (s_k & 7) == 0 && aligned;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        if (vec) {
+        if (vec_rows) {
+            // rows contiguous: 16-byte loads of 8 rows at one k (a wave reads
+            // 128 contiguous bytes per k), transposed into the [row][k] tile
+            const int rr = (t & 7) * 8, kk = (t >> 3) + 32 * h;
+            const int gr = r0 + rr, gk = k0 + kk;
+            bf16x8 v;
+            if (gk < kend && gr + 8 <= R) {
+                v = *reinterpret_cast<const bf16x8 *>(P + gr + gk * s_k);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = (gk < kend && gr + e < R) ? P[(gr + e) + gk * s_k] : (__bf16)0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) S[rr + e][kk] = v[e];
+        } else if (vec) {
             // thread -> (row, 8 consecutive k): 16-byte loads along k
             const int r = (t >> 3) + 32 * h, kk = (t & 7) * 8;
             const int gr = r0 + r, gk = k0 + kk;
