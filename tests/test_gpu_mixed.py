@@ -38,16 +38,35 @@ def test_mixed_sod_error_vs_fp64_oracle():
 def test_mixed_conserves_mass_and_energy():
     mesh = nf.Mesh.box(10, jitter=0.15, seed=12)
     U0 = sod_state(mesh)
-    g, _ = _run(mesh, sod_config(precision="mixed"), U0, 40)
     vol = mesh.fv_grid()["volume"][: mesh.n_cells]
-    U = g.get_solver_data().get_solution()
-    # Each interior face flux is evaluated bit-identically by both cells and
-    # summed in FP64; the slip walls are axis-aligned planes (only interior
-    # nodes are jittered), where the reflected ghost gives an exactly zero
-    # mass and energy flux.  What remains is the FP64 rounding of the update
-    # U + dt/vol R and of the volume-weighted sums.
-    assert abs(vol @ U[:, 0] - vol @ U0[:, 0]) < 1e-13 * (vol @ U0[:, 0])
-    assert abs(vol @ U[:, 4] - vol @ U0[:, 4]) < 1e-13 * (vol @ U0[:, 4])
+    drift = {}
+    for prec in ("fp64", "mixed"):
+        g, _ = _run(mesh, sod_config(precision=prec), U0, 40)
+        U = g.get_solver_data().get_solution()
+        drift[prec] = [abs(vol @ U[:, k] - vol @ U0[:, k]) / (vol @ U0[:, k]) for k in (0, 4)]
+    print("relative drift of mass and energy after 40 steps:", drift)
+    # Interior faces cancel exactly in both modes (each face flux is evaluated
+    # bit-identically by its two cells and summed in FP64).  The slip walls
+    # leak at the rounding level of the reflected ghost state: FP64 there,
+    # FP32 in mixed mode (the unit normal S/|S| is an FP32 quantity).
+    assert max(drift["fp64"]) < 1e-13
+    assert max(drift["mixed"]) < 1e-9
+
+
+def test_mixed_interior_fluxes_cancel_exactly():
+    """At rest (u = 0) next to slip walls the wall mass and energy fluxes are
+    exactly zero, so the cell flux balances sum to exactly zero iff each
+    interior face flux enters its two cells with opposite signs bit for bit.
+    The per-cell balances are FP64 sums of FP32 fluxes of similar magnitude,
+    so math.fsum of them is exact."""
+    import math
+    mesh = nf.Mesh.box(10, jitter=0.15, seed=12)
+    g = nf.EulerSolver(sod_config(precision="mixed"), mesh)
+    g.set_state(sod_state(mesh))
+    g.eval_flux_balance()
+    R = g.get_solver_data().get_flux_balance()
+    assert np.abs(R[:, 0]).max() > 0
+    assert math.fsum(R[:, 0]) == 0.0 and math.fsum(R[:, 4]) == 0.0
 
 
 def test_mixed_farfield_pulse():
