@@ -29,15 +29,30 @@ SEED = 20260101
 
 
 def bytes_model(N, F):
-    """Algorithmic (compulsory) bytes of the two sweeps of one stage, FP64:
-    sweep 1 reads U (40N), volumes (8N), face records (80F) and writes the
-    limited slopes (120N); sweep 2 reads U, slopes, face records, volumes,
-    U_old (stages 2-3) and writes U'."""
+    """Algorithmic bytes of one stage, FP64: the compulsory traffic of a
+    two-sweep stage that streams face records -- sweep 1 reads U (40N),
+    volumes (8N), face records (80F) and writes limited slopes (120N); sweep 2
+    reads U, slopes, face records, volumes, U_old (stages 2-3) and writes U'.
+    These are the bytes `roofline.achieved` is priced on; the kernels here
+    recompute geometry from the nodes and move fewer (`roofline.traffic`)."""
     sweep1 = 168 * N + 80 * F
     sweep2_s1 = 208 * N + 80 * F        # stage 1: no U_old read
     sweep2_s23 = 248 * N + 80 * F
     step = 1232 * N + 480 * F
     return sweep1, sweep2_s1, sweep2_s23, step
+
+
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the 128^3 workload."""
+    path = os.path.join(ROOT, "profiles", "sweep_traffic_128cubed.md")
+    if n != 128 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
 
 
 def cpu_baseline(sample_n, steps):
@@ -71,6 +86,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
+    ap.add_argument("--no-graph", action="store_true", help="eager launches in the timed region (profiling)")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="pick the sweep launch configuration with the autotuner first")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -103,12 +121,13 @@ def main():
           file=sys.stderr, flush=True)
 
     solver.set_state(U0)
-    st = solver.run(args.warmup, check_every=args.graph_steps, use_graph=True)
+    tune = solver.autotune(args.autotune) if args.autotune else None
+    st = solver.run(args.warmup, check_every=args.graph_steps, use_graph=not args.no_graph)
     # timed region: K steps, graph replays; nf_run synchronises before return
     solver.set_state(U0)
     solver.run(1, use_graph=False)  # first step has the zero-U_old quirk; excluded
     t0 = time.perf_counter()
-    st = solver.run(args.steps, check_every=args.graph_steps, use_graph=True)
+    st = solver.run(args.steps, check_every=args.graph_steps, use_graph=not args.no_graph)
     wall = time.perf_counter() - t0
     ms_gpu = st["elapsed_ms"]
     assert st["steps_done"] == args.steps, st
@@ -128,7 +147,9 @@ def main():
     per_launch = {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0}.get(dom, s1)
     achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.n),
+                "traffic_source": "profiles/sweep_traffic_128cubed.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                "kernel": dom,
                 "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
                 "algorithmic_bytes_per_launch": int(per_launch)}
     step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
@@ -152,7 +173,8 @@ def main():
         "data": "synthetic (generated jittered Kuhn mesh, Gaussian pulse)",
         "config": {"workload": f"plain FP64 TVD-RK3 step, unit cube {args.n}^3 hex->6 tets "
                                f"({N} cells), jitter 0.15, Mach 0.5 + pulse, farfield, CFL 0.5",
-                   "cells": N, "faces": F, "graph_steps": args.graph_steps, "cell_order": args.order},
+                   "cells": N, "faces": F, "graph_steps": args.graph_steps, "cell_order": args.order,
+                   "launch": tune},
         "step_roofline_frac": round(step_frac, 4),
         "kernels": kern,
         "roofline": roofline,
