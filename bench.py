@@ -82,8 +82,8 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="hexes per side")
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=48)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-sample-n", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
     ap.add_argument("--no-graph", action="store_true", help="eager launches in the timed region (profiling)")

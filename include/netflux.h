@@ -151,9 +151,11 @@ typedef struct nf_run_opts {
     double residual_drop;         /* <= 0: run all steps                   */
     int32_t check_every;          /* steps per graph replay (>=1)          */
     int32_t use_graph;            /* 1: hipGraph replay, 0: eager launches */
-    /* stall fallback for warm-started instances (0: off): checked between
-     * replays, an instance whose residual did not fall below stall_ratio
-     * times its value stall_window steps earlier restarts from freestream */
+    /* stall fallback for warm-started instances (0: off): once per warm
+     * start, at the first check between replays that is at least
+     * stall_window steps after it, an instance whose residual did not fall
+     * below stall_ratio times its value stall_window steps earlier restarts
+     * from freestream; otherwise the warm start is accepted */
     int32_t stall_window;
     double stall_ratio;
 } nf_run_opts;
@@ -213,7 +215,9 @@ int nf_inst_set_state(nf_ctx *ctx, int32_t inst, const double *U, const double *
 int nf_inst_get_state(nf_ctx *ctx, int32_t inst, double *U);
 int nf_inst_residuals(nf_ctx *ctx, int32_t inst, double *out, int32_t max, int32_t *n_out);
 /* Runs every instance until it converges (residual_drop) or n_steps steps
- * are done; stats[n] per instance, elapsed_ms is the batch wall time. */
+ * are done; stats[n] per instance, elapsed_ms is the batch wall time.  With
+ * a residual drop, an instance that converged in an earlier call stays
+ * stopped (runs can continue in chunks) until its state is set again. */
 int nf_run_batch(nf_ctx *ctx, const nf_run_opts *opts, nf_run_stats *stats);
 
 /* The drop of instance `inst` is measured against `ref` instead of its first
@@ -234,6 +238,11 @@ typedef struct nf_tune_result {
 } nf_tune_result;
 int nf_autotune(nf_ctx *ctx, int32_t steps, nf_tune_result *out);
 int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap);
+/* Switches the face-sweep arithmetic of a context (NF_PREC_*), keeping the
+ * FP64 states, run counters and residual histories: e.g. mixed precision
+ * down to its FP32 noise floor, then FP64 to a tighter drop.  Primvars and
+ * dt are recomputed from the states; converged flags are cleared. */
+int nf_set_precision(nf_ctx *ctx, int32_t precision);
 
 /* ---- learned warm start ---------------------------------------------------
  * MLP 6 -> 128 -> 128 -> 128 -> 5 with GELU: per-cell features (Mach, bump

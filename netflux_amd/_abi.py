@@ -119,6 +119,7 @@ SIGNATURES = [
     ("nf_run_batch", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     ("nf_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(TuneResult)]),
     ("nf_set_launch", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("nf_set_precision", C.c_int, [C.c_void_p, C.c_int32]),
     ("nf_inst_set_residual_ref", C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     ("nf_eval_residuals", C.c_int, [C.c_void_p, f64p]),
     ("nf_inst_features", C.c_int, [C.c_void_p, C.c_int32, f64p, f32p]),

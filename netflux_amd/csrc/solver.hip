@@ -103,14 +103,16 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_res(const double *red, const
 }
 
 // op 0: clear the run flags of every instance (err, stop, converged, error
-// counters); op 1: set dt of instance `which`
+// counters); op 2: the same, but a converged instance stays converged and
+// stopped; op 1: set dt of instance `which`
 __global__ void k_flags(InstState *st, int n, int op, int which, double dt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (op == 0) {
+    if (op == 0 || op == 2) {
+        const int keep = op == 2 && st[i].converged;
         st[i].err = 0;
-        st[i].stop = 0;
-        st[i].converged = 0;
+        st[i].stop = keep;
+        st[i].converged = keep;
         st[i].bad_flux = 0;
         st[i].bad_state = 0;
     } else if (i == which) {
@@ -443,9 +445,9 @@ int enqueue_timestep(nf_ctx *ctx) {
     return reduce_dt(ctx, 0);
 }
 
-int clear_flags(nf_ctx *ctx) {
-    hipLaunchKernelGGL(k_flags, dim3((ctx->n_inst + 63) / 64), dim3(64), 0, ctx->stream, ctx->st, ctx->n_inst, 0,
-                       0, 0.0);
+int clear_flags(nf_ctx *ctx, bool keep_converged = false) {
+    hipLaunchKernelGGL(k_flags, dim3((ctx->n_inst + 63) / 64), dim3(64), 0, ctx->stream, ctx->st, ctx->n_inst,
+                       keep_converged ? 2 : 0, 0, 0.0);
     HIPC(hipGetLastError());
     return NF_OK;
 }
@@ -886,7 +888,9 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
     int rc;
     const int K = std::max(1, o->check_every);
     const int n = ctx->n_inst;
-    if ((rc = clear_flags(ctx))) return rc;
+    // with a residual drop, an instance that already reached it stays stopped
+    // (a run can be continued in chunks); its state must be set to run again
+    if ((rc = clear_flags(ctx, o->residual_drop > 0.0))) return rc;
     std::vector<bool> need(n);
     for (int i = 0; i < n; ++i) need[i] = !ctx->inst[i].dt_valid;
     if (std::any_of(need.begin(), need.end(), [](bool b) { return b; }) && (rc = timestep_subset(ctx, need)))
@@ -939,15 +943,18 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
             std::vector<InstState> h;
             if ((rc = read_states(ctx, h))) return rc;
             if (stall) {
+                // one check per warm start, at the first sync point at least
+                // stall_window steps after it: accepted or replaced
                 std::vector<bool> reset(n, false);
                 for (int i = 0; i < n; ++i) {
                     const int k = h[i].nstep;
-                    if (!ctx->inst[i].warm || h[i].stop || k <= o->stall_window || k > ctx->hist_cap) continue;
+                    if (!ctx->inst[i].warm || k <= o->stall_window || k > ctx->hist_cap) continue;
                     double r_now = 0, r_then = 0;
                     const double *hp = ctx->hist + (size_t)i * ctx->hist_cap;
                     HIPC(hipMemcpy(&r_now, hp + k - 1, sizeof(double), hipMemcpyDeviceToHost));
                     HIPC(hipMemcpy(&r_then, hp + k - 1 - o->stall_window, sizeof(double), hipMemcpyDeviceToHost));
-                    if (!(r_now <= o->stall_ratio * r_then)) reset[i] = true;
+                    if (r_now <= o->stall_ratio * r_then || h[i].converged) ctx->inst[i].warm = false;
+                    else reset[i] = true;
                 }
                 if (std::any_of(reset.begin(), reset.end(), [](bool b) { return b; })) {
                     for (int i = 0; i < n; ++i) {
@@ -1287,6 +1294,41 @@ int nf_inst_warm_start(nf_ctx *ctx, int32_t i, nf_mlp *net, const double *cp, in
     // U_old = U: the first RK3 step must not scale a warm start
     if ((rc = after_state_set(ctx, i, false))) return rc;
     I.warm = true;
+    return NF_OK;
+}
+
+int nf_set_precision(nf_ctx *ctx, int32_t precision) {
+    if (!ctx || (precision != NF_PREC_FP64 && precision != NF_PREC_MIXED)) return NF_E_ARG;
+    const bool mixed = precision == NF_PREC_MIXED;
+    if (mixed == ctx->mixed) return NF_OK;
+    int rc;
+    const size_t NC = ctx->Ncell;
+    if (mixed && !ctx->V32) {
+        if ((rc = dalloc(ctx, &ctx->V32, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV32, 20 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
+    }
+    if (!mixed && !ctx->V) {
+        if ((rc = dalloc(ctx, &ctx->V, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->FV, 20 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+    }
+    HIPC(hipStreamSynchronize(ctx->stream));
+    ctx->mixed = mixed;
+    ctx->prm.precision = precision;
+    if (ctx->graph) {  // captured with the other precision's kernels and buffers
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    // primvars of the current states in the new precision; dt from them;
+    // every instance runs again (converged flags cleared)
+    if ((rc = clear_flags(ctx))) return rc;
+    if ((rc = prim_all(ctx))) return rc;
+    for (Inst &I : ctx->inst) I.dt_valid = false;
+    ctx->aux_valid = false;
+    HIPC(hipStreamSynchronize(ctx->stream));
     return NF_OK;
 }
 

@@ -276,6 +276,10 @@ class EulerSolver:
     def set_launch(self, block, xcd_remap=True):
         _check(A.lib().nf_set_launch(self._ctx, block, int(xcd_remap)), self._ctx, "nf_set_launch")
 
+    def set_precision(self, precision):
+        p = {"fp64": A.NF_PREC_FP64, "mixed": A.NF_PREC_MIXED}.get(precision, precision)
+        _check(A.lib().nf_set_precision(self._ctx, p), self._ctx, "nf_set_precision")
+
 
 def _autotune(ctx, steps):
     r = A.TuneResult()
@@ -377,6 +381,10 @@ class BatchSolver:
 
     def set_launch(self, block, xcd_remap=True):
         _check(A.lib().nf_set_launch(self._ctx, block, int(xcd_remap)), self._ctx, "nf_set_launch")
+
+    def set_precision(self, precision):
+        p = {"fp64": A.NF_PREC_FP64, "mixed": A.NF_PREC_MIXED}.get(precision, precision)
+        _check(A.lib().nf_set_precision(self._ctx, p), self._ctx, "nf_set_precision")
 
 
 class Driver:

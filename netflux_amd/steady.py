@@ -60,9 +60,17 @@ def mesh_of(case):
     return S.Mesh.bump_channel(case.nx, case.ny, case.nz, h=case.h)
 
 
-def config_of(case, precision="fp64", local_time_step=True, cfl=0.5):
+# Steady defaults: CFL 1.0 is stable for the unlimited scheme (1.5 is not);
+# the Barth limiter stalls these flows near a 5e-4 drop, so 1e-8 targets use
+# the unlimited second-order reconstruction (limiter NONE of the reference).
+STEADY_CFL = 1.0
+STEADY_LIMITER = "NONE"
+
+
+def config_of(case, precision="fp64", local_time_step=True, cfl=STEADY_CFL, limiter=STEADY_LIMITER):
     return S.Config(density_fs=1.0, velocity_fs=(case.mach, 0.0, 0.0), pressure_fs=1.0 / GAMMA, CFL=cfl,
-                    rk3_zero_old_first_step=False, precision=precision, local_time_step=local_time_step)
+                    limiter=limiter, rk3_zero_old_first_step=False, precision=precision,
+                    local_time_step=local_time_step)
 
 
 def freestream_state(cfg, n):
@@ -99,11 +107,13 @@ def wall_pressure(mesh, U):
 class BatchRun:
     """One batched context over `cases`; solve() runs them all to the drop."""
 
-    def __init__(self, cases, precision="mixed", local_time_step=True, autotune_steps=0):
+    def __init__(self, cases, precision="mixed", local_time_step=True, autotune_steps=0, cfl=STEADY_CFL,
+                 limiter=STEADY_LIMITER):
         self.cases = cases
         self.meshes = [mesh_of(c) for c in cases]
-        self.cfgs = [config_of(c, precision, local_time_step) for c in cases]
+        self.cfgs = [config_of(c, precision, local_time_step, cfl, limiter) for c in cases]
         self.batch = S.BatchSolver(self.cfgs, self.meshes)
+        self.precision = precision
         self.tune = None
         self.autotune_steps = autotune_steps
 
@@ -112,9 +122,13 @@ class BatchRun:
             U = freestream_state(cfg, N)
             self.batch.set_state(i, U, U)
 
-    def solve(self, drop=1e-8, max_steps=200000, check_every=200, net=None, stall_window=0, stall_ratio=0.0):
-        """Returns per-instance stats and the wall time (s) of the solve."""
+    def solve(self, drop=1e-8, max_steps=200000, check_every=200, net=None, stall_window=0, stall_ratio=0.0,
+              chunk_steps=10000, switch_drop=1e-6):
+        """Returns per-instance stats and the wall time (s) of the solve.
+        Mixed precision runs to `switch_drop` (above its FP32 noise floor),
+        then the same batch continues in FP64 to `drop`."""
         b = self.batch
+        b.set_precision(self.precision)
         self.reset_freestream()
         r_fs = b.eval_residuals()
         for i, r in enumerate(r_fs):
@@ -126,12 +140,49 @@ class BatchRun:
         if net is not None:
             for i, c in enumerate(self.cases):
                 fell[i] = b.warm_start(i, net, (c.mach, c.h))
-        st = b.run(max_steps, residual_drop=drop, check_every=check_every, stall_window=stall_window,
-                   stall_ratio=stall_ratio)
+        st = None
+        done = 0
+        phases = [("mixed", switch_drop), ("fp64", drop)] if self.precision == "mixed" and switch_drop > drop \
+            else [(self.precision, drop)]
+        phase = 0
+        b.set_precision(phases[0][0])
+        while done < max_steps:
+            k = min(chunk_steps, max_steps - done)
+            part = b.run(k, residual_drop=phases[phase][1], check_every=check_every, stall_window=stall_window,
+                         stall_ratio=stall_ratio)
+            if st is None:
+                st = part
+            else:
+                for s0, s1 in zip(st, part):
+                    s0["steps_done"] += s1["steps_done"]
+                    s0["converged"] = s1["converged"]
+                    s0["fell_back"] = int(s0["fell_back"] or s1["fell_back"])
+                    s0["res_last"] = s1["res_last"] if s1["steps_done"] else s0["res_last"]
+            done += k
+            n_conv = sum(s["converged"] for s in st)
+            # an unconverged instance whose residual fell by less than 30% over
+            # the chunk has reached the noise floor of the current precision
+            stalled = sum(1 for s, p in zip(st, part) if not s["converged"] and p["steps_done"] and
+                          p["res_last"] > 0.7 * s.get("res_prev", np.inf))
+            for s, p in zip(st, part):
+                if p["steps_done"]:
+                    s["res_prev"] = p["res_last"]
+            _log(f"[solve] {done} steps, {n_conv}/{b.n} converged ({phases[phase][0]}, drop "
+                 f"{phases[phase][1]:.0e}), {time.perf_counter() - t0:.1f} s")
+            if n_conv == b.n or (phase + 1 < len(phases) and n_conv + stalled == b.n):
+                if phase + 1 == len(phases):
+                    break
+                phase += 1
+                b.set_precision(phases[phase][0])
+                _log(f"[solve] switching to {phases[phase][0]} after {done} steps")
+                for s0 in st:
+                    s0["converged"] = 0
+                    s0.pop("res_prev", None)
         wall = time.perf_counter() - t0
         for i in range(b.n):
             st[i]["fell_back"] = int(st[i]["fell_back"] or fell[i])
             st[i]["r_ref"] = float(r_fs[i])
+            st[i].pop("res_prev", None)
         return st, wall
 
     def solutions(self):
@@ -142,12 +193,14 @@ class BatchRun:
         return self.batch.features(i, (c.mach, c.h))
 
 
-def training_set(run, per_case, seed=0):
+def training_set(run, per_case, stats, seed=0):
     """(features, primitive variables) of `per_case` random cells of each
-    converged solution of a BatchRun."""
+    converged, finite solution of a BatchRun."""
     rng = np.random.default_rng(seed)
     xs, ys = [], []
     for i, U in enumerate(run.solutions()):
+        if not stats[i]["converged"] or not np.all(np.isfinite(U)):
+            continue
         x = run.features(i)
         V = cons_to_prim(U)
         pick = rng.choice(len(U), size=min(per_case, len(U)), replace=False)
@@ -156,8 +209,16 @@ def training_set(run, per_case, seed=0):
     return np.concatenate(xs), np.concatenate(ys)
 
 
-def train_network(x, y, epochs=30, batch=4096, lr=2e-3, lr_decay=0.92, seed=0):
+def train_network(x, y, epochs=200, batch=2048, lr=2e-3, lr_decay=0.98, seed=0):
+    """Inputs normalised by their mean / standard deviation; outputs by
+    their mean and max(std, 0.01): the spanwise velocity is zero up to
+    discretisation noise, which a unit-variance target would amplify."""
     net = Mlp(seed)
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    sx = x.std(0)
+    net.set_norm(x.mean(0), np.where(sx > 0, 1.0 / np.where(sx > 0, sx, 1.0), 1.0), y.mean(0),
+                 np.maximum(y.std(0), 0.01))
     stats = net.train(x, y, epochs=epochs, batch=batch, lr=lr, lr_decay=lr_decay, seed=seed)
     return net, stats
 
@@ -169,20 +230,30 @@ def _log(msg):
 
 def _summary(stats):
     steps = [s["steps_done"] for s in stats]
+    drops = [s["res_last"] / s["r_ref"] if s.get("r_ref") else float("nan") for s in stats]
     return {"converged": int(sum(s["converged"] for s in stats)), "n": len(stats),
+            "final_drop_max": float(np.max(drops)), "final_drop_median": float(np.median(drops)),
             "steps_min": int(min(steps)), "steps_median": float(np.median(steps)), "steps_max": int(max(steps)),
             "fell_back": int(sum(s.get("fell_back", 0) for s in stats))}
 
 
 def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60000, check_every=200,
-          train_cases=0, train_cells=2e5, per_case=4000, epochs=30, train_drop=1e-6, plain=True,
-          plain_lts=False):
+          train_cases=0, train_cells=2e5, per_case=4000, epochs=200, train_drop=1e-6, plain=True,
+          plain_lts=False, partial=None, train_max_steps=60000):
     """Plain FP64 solves one by one (the reference's global time step unless
     plain_lts), the batched mixed-precision solve with local time steps, and
     (train_cases > 0) the same batch from a learned warm start trained on
     train_cases plain solutions of train_cells cells."""
     cases = sample_cases(n_cases, seed, cells_lo, cells_hi)
     rep = {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "plain_lts": plain_lts}
+
+    def save():
+        if partial:
+            import json
+            with open(partial, "w") as f:
+                json.dump(rep, f)
+
+    save()
     plain_U, plain_wall, plain_stats = [], [], []
     if plain:
         for i, c in enumerate(cases):
@@ -195,40 +266,52 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
                  f"{wall:.2f} s")
             del run
         rep["plain_fp64"] = _summary(plain_stats) | {"wall_total_s": sum(plain_wall),
-                                                       "per_instance_s": sum(plain_wall) / n_cases}
+                                                       "per_instance_s": sum(plain_wall) / n_cases,
+                                                       "steps": [s["steps_done"] for s in plain_stats],
+                                                       "wall_s": plain_wall}
+        save()
 
     def compare(sols):
-        out = []
+        """Wall pressure and state against the plain FP64 converged solutions."""
+        dp, du, dl = [], [], []
         for i, (c, U) in enumerate(zip(cases, sols)):
             if not plain_U:
                 break
             m = mesh_of(c)
             _, p = wall_pressure(m, U)
             _, p0 = wall_pressure(m, plain_U[i])
-            out.append(float(np.max(np.abs(p - p0))))
-        return {"wall_p_maxabs_max": max(out), "wall_p_maxabs_median": float(np.median(out))} if out else {}
+            dp.append(float(np.max(np.abs(p - p0))))
+            du.append(float(np.max(np.abs(U - plain_U[i]))))
+            dl.append(float(np.linalg.norm(U - plain_U[i]) / np.linalg.norm(plain_U[i])))
+        if not dp:
+            return {}
+        return {"wall_p_maxabs_max": max(dp), "wall_p_maxabs_median": float(np.median(dp)),
+                "U_maxabs_max": max(du), "U_relL2_max": max(dl)}
 
     run = BatchRun(cases, precision="mixed", autotune_steps=20)
     st, wall = run.solve(drop, max_steps, check_every)
     _log(f"[batch mixed] {_summary(st)} {wall:.2f} s, tune {run.tune}")
-    rep["batch_mixed"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases, "tune": run.tune} | \
-        compare(run.solutions())
+    rep["batch_mixed"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases, "tune": run.tune,
+                                         "steps": [s["steps_done"] for s in st]} | compare(run.solutions())
+    save()
     if train_cases:
         tcases = sample_cases(train_cases, seed + 1000, train_cells, train_cells)
         t0 = time.perf_counter()
         trun = BatchRun(tcases, precision="fp64")
-        tst, twall = trun.solve(train_drop, max_steps, check_every)
-        x, y = training_set(trun, per_case, seed)
+        tst, twall = trun.solve(train_drop, train_max_steps, check_every)
+        x, y = training_set(trun, per_case, tst, seed)
         del trun
         _log(f"[train data] {_summary(tst)} {twall:.2f} s, {len(x)} samples")
         net, tstats = train_network(x, y, epochs=epochs)
         _log(f"[train] {tstats}")
         rep["training"] = {"solves": _summary(tst), "solve_wall_s": twall, "samples": int(len(x)),
                            "train": tstats, "total_s": time.perf_counter() - t0}
+        save()
         st, wall = run.solve(drop, max_steps, check_every, net=net, stall_window=4000, stall_ratio=0.5)
         _log(f"[batch mixed warm] {_summary(st)} {wall:.2f} s")
-        rep["batch_mixed_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases} | \
-            compare(run.solutions())
+        rep["batch_mixed_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases,
+                                                   "steps": [s["steps_done"] for s in st]} | compare(run.solutions())
+        save()
     return rep
 
 
@@ -246,12 +329,14 @@ def main(argv=None):
     ap.add_argument("--train-cases", type=int, default=0)
     ap.add_argument("--train-cells", type=float, default=2e5)
     ap.add_argument("--per-case", type=int, default=4000)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-lts", action="store_true")
+    ap.add_argument("--partial", default=None, help="write the report after each phase to this file")
     a = ap.parse_args(argv)
     rep = study(a.cases, a.seed, a.cells_lo, a.cells_hi, a.drop, a.max_steps, a.check_every, a.train_cases,
-                a.train_cells, a.per_case, a.epochs, plain=not a.no_plain, plain_lts=a.plain_lts)
+                a.train_cells, a.per_case, a.epochs, plain=not a.no_plain, plain_lts=a.plain_lts,
+                partial=a.partial)
     print(json.dumps(rep))
 
 

@@ -132,3 +132,6 @@ def test_instances_stop_independently_at_residual_drop():
         s.run(stats[i]["steps_done"], check_every=1)
         np.testing.assert_array_equal(b.get_solution(i), s.get_solver_data().get_solution())
     assert stats[a]["converged"] == 1 and stats[a]["steps_done"] <= k + 1
+    # a converged instance stays stopped when the run continues with a drop
+    again = b.run(8, residual_drop=drop, check_every=4)
+    assert again[a]["steps_done"] == 0 and again[a]["converged"] == 1
