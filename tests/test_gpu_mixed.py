@@ -77,3 +77,16 @@ def test_mixed_farfield_pulse():
     ora.set_state(U0, uold_equals_u=True)
     ora.run(30)
     assert rel_l2(g.get_solver_data().get_solution(), ora.get("U")) < 1e-5
+
+
+def test_precision_switch_continues_in_fp64():
+    """Mixed for 10 steps, then FP64 for 10 in the same context, equals a FP64
+    context started from the mixed state (primvars and dt recomputed from U)."""
+    mesh = nf.Mesh.box(8, jitter=0.1, seed=21, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    a, _ = _run(mesh, pulse_config(precision="mixed"), U0, 10)
+    U10 = a.get_solver_data().get_solution()
+    a.set_precision("fp64")
+    a.run(10, use_graph=True, check_every=5)
+    b, _ = _run(mesh, pulse_config(precision="fp64"), U10, 10)
+    np.testing.assert_array_equal(a.get_solver_data().get_solution(), b.get_solver_data().get_solution())
