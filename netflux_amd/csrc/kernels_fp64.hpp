@@ -44,9 +44,6 @@ namespace nf {
 #ifndef NF_B_GROUP
 #define NF_B_GROUP 2
 #endif
-#ifndef NF_B_NODES_FIRST
-#define NF_B_NODES_FIRST 1
-#endif
 // waves per SIMD the sweeps are compiled for (0: the compiler's choice)
 #ifndef NF_SWEEP_WAVES
 #define NF_SWEEP_WAVES 0
@@ -86,6 +83,12 @@ struct DevMesh {
     double three, six;       // 3.0 and 6.0, divisors of the centroid / volume formulas
     const double *nodes;
     const uint32_t *cnodes;
+    // per 64-cell group: its distinct nodes' coordinates, SoA, entries
+    // [grp_off[g], grp_off[g+1]); cell_ln: the group-local entry of each
+    // cell's four nodes (4 x u16)
+    const uint32_t *grp_off;
+    const double *gx, *gy, *gz;
+    const uint2 *cell_ln;
     const uint32_t *nbr;
     const uint32_t *fcode;
     const uint32_t *blk_inst;  // [blocks] instance of each 256-cell block (32-bit: scalar loads)
@@ -273,9 +276,7 @@ struct CellGeo {
     Rcp ivol;
 };
 
-// Node loads of cell c.  Issued before the state gathers of a sweep: the
-// memory counter retires loads in order, so waiting for the nodes then does
-// not wait for the gathers.
+// Node loads of cell c from the node array (boundary-face kernels).
 __device__ __forceinline__ void load_nodes(const DevMesh &m, uint32_t c, CellGeo &g) {
     const uint4 q = *reinterpret_cast<const uint4 *>(m.cnodes + 4 * (size_t)c);
     auto node = [&](uint32_t n) {
@@ -302,36 +303,99 @@ __device__ __forceinline__ void load_cell(const DevMesh &m, uint32_t c, CellGeo 
     cell_geo(m, g);
 }
 
-// The four nodes of each lane's cell staged in LDS, [12][B] (node q,
-// coordinate d at row 3q+d); every lane reads only its own column, so no
-// barrier is needed, and a wave's reads of one row hit 64 distinct banks.
-// Selecting a face's nodes by index from LDS replaces register selects.
+// Node coordinates in LDS, rows x, y, z of `stride` entries each; l[q] is
+// the entry of the lane's q-th cell node.  Selecting a face's nodes by index
+// from LDS replaces register selects.
+struct NodeView {
+    const double *lds;
+    uint32_t l[4];
+    uint32_t stride;
+};
+__device__ __forceinline__ V3d nv_node(const NodeView &v, uint32_t q) {
+    const uint32_t i = q == 0 ? v.l[0] : (q == 1 ? v.l[1] : (q == 2 ? v.l[2] : v.l[3]));
+    return {v.lds[i], v.lds[v.stride + i], v.lds[2 * v.stride + i]};
+}
+
+// Per-lane staging (boundary-face kernels): lane t writes its four nodes at
+// entries q*B + t; only its own entries are read, so no barrier is needed.
 template <int B>
-__device__ __forceinline__ void stage_nodes(double *lds, const CellGeo &g) {
+__device__ __forceinline__ NodeView stage_nodes(double *lds, const CellGeo &g) {
     const uint32_t t = threadIdx.x;
     const V3d P[4] = {g.p0, g.p1, g.p2, g.p3};
+    NodeView v{lds, {t, B + t, 2 * B + t, 3 * B + t}, 4 * B};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        lds[(3 * q + 0) * B + t] = P[q].x;
-        lds[(3 * q + 1) * B + t] = P[q].y;
-        lds[(3 * q + 2) * B + t] = P[q].z;
+        lds[q * B + t] = P[q].x;
+        lds[4 * B + q * B + t] = P[q].y;
+        lds[8 * B + q * B + t] = P[q].z;
     }
+    return v;
+}
+
+// Per-wave node table (sweeps): the wave's 64 cells (one group) load their
+// distinct nodes once, coalesced, into the wave's LDS slice (at most 256
+// entries); each lane then takes its cell's nodes from there.  The loads are
+// issued here, before the sweep's state gathers, so that waiting for them
+// does not wait for the gathers (memory counters retire in order).  p is the
+// lane's device cell position (padding included).  Only lanes of the same
+// wave exchange data through the slice, so no block barrier is needed.
+struct NodeTableLoad {
+    double x, y, z;
+    uint32_t n;
+    uint2 q;
+};
+__device__ __forceinline__ NodeTableLoad node_table_issue(const DevMesh &m, uint32_t p) {
+    const uint32_t gi = p >> 6, lane = p & 63u;
+    const uint32_t o0 = m.grp_off[gi], n = m.grp_off[gi + 1] - o0;
+    NodeTableLoad t;
+    t.n = n;
+    t.q = m.cell_ln[p];
+    // up to 256 nodes: four per lane at most; the first 64 here, the rest
+    // (rare: only very irregular orderings) in node_table_finish
+    const uint32_t i = lane < n ? lane : 0;
+    t.x = m.gx[o0 + i];
+    t.y = m.gy[o0 + i];
+    t.z = m.gz[o0 + i];
+    return t;
 }
 template <int B>
-__device__ __forceinline__ V3d lds_node(const double *lds, uint32_t q) {
-    const uint32_t t = threadIdx.x;
-    return {lds[(3 * q + 0) * B + t], lds[(3 * q + 1) * B + t], lds[(3 * q + 2) * B + t]};
+__device__ __forceinline__ NodeView node_table_finish(const DevMesh &m, uint32_t p, const NodeTableLoad &t,
+                                                      double *lds, CellGeo &g) {
+    const uint32_t lane = p & 63u, wv = threadIdx.x >> 6;
+    double *w = lds + wv * 768;  // [3][256] per wave
+    if (lane < t.n) {
+        w[lane] = t.x;
+        w[256 + lane] = t.y;
+        w[512 + lane] = t.z;
+    }
+    if (t.n > 64) {  // uniform per wave
+        const uint32_t o0 = m.grp_off[p >> 6];
+        for (uint32_t i = lane + 64; i < t.n; i += 64) {
+            w[i] = m.gx[o0 + i];
+            w[256 + i] = m.gy[o0 + i];
+            w[512 + i] = m.gz[o0 + i];
+        }
+    }
+    // LDS operations of one wave complete in order; the fences keep the
+    // compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    NodeView v{w, {t.q.x & 0xFFFFu, t.q.x >> 16, t.q.y & 0xFFFFu, t.q.y >> 16}, 256};
+    g.p0 = nv_node(v, 0);
+    g.p1 = nv_node(v, 1);
+    g.p2 = nv_node(v, 2);
+    g.p3 = nv_node(v, 3);
+    return v;
 }
 
 // Face normal oriented i->j (triangle formula of src/geometry/
 // GeometryUtilities.cpp:318-324, flip of src/geometry/FV_Grid.cpp:247-250)
 // and centroid-to-face vector (triangle_calc_centroid, :326-329) from the
 // face element's node order.
-template <int B>
-__device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, const double *lds, uint32_t byte,
+__device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, const NodeView &nv, uint32_t byte,
                                          V3d &S, V3d *r) {
-    const V3d a = lds_node<B>(lds, byte & 3u), b = lds_node<B>(lds, (byte >> 2) & 3u),
-              c = lds_node<B>(lds, (byte >> 4) & 3u);
+    const V3d a = nv_node(nv, byte & 3u), b = nv_node(nv, (byte >> 2) & 3u), c = nv_node(nv, (byte >> 4) & 3u);
     S = vscl(0.5, vcross(vsub(b, a), vsub(c, a)));
     if (byte & 0x40u) S = vscl(-1, S);
     if (r) {
@@ -405,9 +469,9 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
     const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
     CellGeo g;
     load_cell(m, c, g);
-    stage_nodes<BLOCK>(lds, g);
+    const NodeView nv = stage_nodes<BLOCK>(lds, g);
     V3d S;
-    face_geo<BLOCK>(m, g, lds, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    face_geo(m, g, nv, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
     double Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
@@ -445,12 +509,11 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
+    __builtin_amdgcn_sched_barrier(0);
     double Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
-    CellGeo g;
-    load_nodes(m, c, g);
-    __builtin_amdgcn_sched_barrier(0);
     // neighbour (or ghost) states; NF_A_PREFETCH = 1 loads variable k + 1
     // while variable k is processed instead of all five up front, which
     // trades memory-level parallelism for registers
@@ -469,11 +532,12 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 #pragma unroll
             for (int s = 0; s < 4; ++s) Vn[s][k] = nsrc[s][k * nstride[s]];
     }
+    CellGeo g;
+    const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
     cell_geo(m, g);
-    stage_nodes<B>(lds, g);
     V3d S[4], r[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo<B>(m, g, lds, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+    for (int s = 0; s < 4; ++s) face_geo(m, g, nv, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     __builtin_amdgcn_sched_barrier(0);
 
     if (ORDER == 0) {
@@ -572,11 +636,10 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArg
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
     // face values of both sides, loaded NF_B_GROUP faces at a time (2, the
     // default, measured 4% faster than issuing all four faces' loads up front)
+    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
+    __builtin_amdgcn_sched_barrier(0);
     CellGeo g;
-    if (NF_B_NODES_FIRST) {
-        load_nodes(m, c, g);
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    NodeView nv;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int g0 = 0; g0 < 4; g0 += NF_B_GROUP) {
@@ -595,16 +658,15 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArg
             }
         }
         if (g0 == 0) {
-            if (!NF_B_NODES_FIRST) load_nodes(m, c, g);
+            nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
             cell_geo(m, g);
-            stage_nodes<B>(lds, g);
         }
 #pragma unroll
         for (int q = 0; q < NF_B_GROUP; ++q) {
             const int s = g0 + q;
             const uint32_t byte = (code >> (8 * s)) & 0xFFu;
             V3d S;
-            face_geo<B>(m, g, lds, byte, S, nullptr);
+            face_geo(m, g, nv, byte, S, nullptr);
             // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
             double UO[5], UN[5], F[5];
             prim_to_cons(Vo4[q], UO);

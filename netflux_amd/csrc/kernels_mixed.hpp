@@ -63,10 +63,8 @@ __device__ __forceinline__ V3f to_f(V3d v) { return {(float)v.x, (float)v.y, (fl
 
 // S (FP32, from FP64 edge vectors of the face's own node order) and, if r is
 // given, the FP32 centroid-to-face vector
-template <int B>
-__device__ __forceinline__ void face_geo_mx(const double *lds, const CellGeo &g, uint32_t byte, V3f &S, V3f *r) {
-    const V3d a = lds_node<B>(lds, byte & 3u), b = lds_node<B>(lds, (byte >> 2) & 3u),
-              c = lds_node<B>(lds, (byte >> 4) & 3u);
+__device__ __forceinline__ void face_geo_mx(const NodeView &nv, const CellGeo &g, uint32_t byte, V3f &S, V3f *r) {
+    const V3d a = nv_node(nv, byte & 3u), b = nv_node(nv, (byte >> 2) & 3u), c = nv_node(nv, (byte >> 4) & 3u);
     const V3f e1 = to_f(vsub(b, a)), e2 = to_f(vsub(c, a));
     S = {0.5f * (e1.y * e2.z - e1.z * e2.y), 0.5f * (e1.z * e2.x - e1.x * e2.z), 0.5f * (e1.x * e2.y - e1.y * e2.x)};
     if (byte & 0x40u) S = {-S.x, -S.y, -S.z};
@@ -135,9 +133,9 @@ __global__ __launch_bounds__(BLOCK) void k_ghost_mx(DevMesh m, GhostArgsMx a) {
     const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
     CellGeo g;
     load_cell(m, c, g);
-    stage_nodes<BLOCK>(lds, g);
+    const NodeView nv = stage_nodes<BLOCK>(lds, g);
     V3f S;
-    face_geo_mx<BLOCK>(lds, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    face_geo_mx(nv, g, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
     float Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
@@ -160,12 +158,11 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
+    __builtin_amdgcn_sched_barrier(0);
     float Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
-    CellGeo g;
-    load_nodes(m, c, g);  // before the gathers: waiting for nodes must not wait for them
-    __builtin_amdgcn_sched_barrier(0);
     float Vn[4][5];
     if (ORDER != 0) {
 #pragma unroll
@@ -177,11 +174,12 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
             for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
         }
     }
+    CellGeo g;
+    const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
     cell_geo(m, g);
-    stage_nodes<B>(lds, g);
     V3f S[4], r[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) face_geo_mx<B>(lds, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
+    for (int s = 0; s < 4; ++s) face_geo_mx(nv, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     const float ivol = (float)(1.0 / g.vol);
 
     if (ORDER == 0) {
@@ -251,8 +249,7 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    CellGeo g;
-    load_nodes(m, c, g);
+    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
     __builtin_amdgcn_sched_barrier(0);
     float Vo4[4][5], Vn4[4][5];
 #pragma unroll
@@ -267,14 +264,15 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
             Vn4[s][k] = src[k * stride];
         }
     }
+    CellGeo g;
+    const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
     cell_geo(m, g);
-    stage_nodes<B>(lds, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t byte = (code >> (8 * s)) & 0xFFu;
         V3f S;
-        face_geo_mx<B>(lds, g, byte, S, nullptr);
+        face_geo_mx(nv, g, byte, S, nullptr);
         float UO[5], UN[5], F[5];
         prim_to_cons_f(Vo4[s], UO);
         prim_to_cons_f(Vn4[s], UN);

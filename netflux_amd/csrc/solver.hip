@@ -214,6 +214,9 @@ struct nf_ctx {
     bool mixed = false;
     double *nodes = nullptr;
     uint32_t *cnodes = nullptr, *nbr = nullptr, *fcode = nullptr;
+    uint32_t *grp_off = nullptr;          // 64-cell group node tables
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    uint2 *cell_ln = nullptr;
     uint32_t *blk_inst = nullptr;
     uint32_t *inst_end = nullptr;
     InstState *st = nullptr;
@@ -273,7 +276,8 @@ int dalloc(nf_ctx *ctx, T **p, size_t n) {
 }
 
 DevMesh devmesh(const nf_ctx *c) {
-    return DevMesh{3.0, 6.0, c->nodes, c->cnodes, c->nbr, c->fcode, c->blk_inst, c->inst_end, c->st};
+    return DevMesh{3.0,         6.0,       c->nodes, c->cnodes, c->grp_off,  c->gx,       c->gy,
+                   c->gz,       c->cell_ln, c->nbr,   c->fcode,  c->blk_inst, c->inst_end, c->st};
 }
 
 const RedTable &red(const nf_ctx *c) { return c->block == 128 ? c->red128 : c->red256; }
@@ -792,6 +796,33 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
         node0 += d->n_nodes;
     }
     if (bface.size() > NBR_INDEX_MASK) return NF_E_ARG;
+    // node tables of the 64-cell groups (one per wave; instances are padded
+    // to 256 cells): distinct nodes in first-use order
+    std::vector<uint32_t> grp_off(1, 0);
+    std::vector<double> gx, gy, gz;
+    std::vector<uint2> cell_ln(cell0);
+    {
+        std::vector<uint32_t> stamp(node0, UINT32_MAX), loc(node0);
+        for (uint32_t gi = 0; gi < cell0 / 64; ++gi) {
+            uint32_t cnt = 0;
+            for (uint32_t p = gi * 64; p < gi * 64 + 64; ++p) {
+                uint32_t l[4];
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t v = cnodes[4 * (size_t)p + q];
+                    if (stamp[v] != gi) {
+                        stamp[v] = gi;
+                        loc[v] = cnt++;
+                        gx.push_back(nodes4[4 * (size_t)v]);
+                        gy.push_back(nodes4[4 * (size_t)v + 1]);
+                        gz.push_back(nodes4[4 * (size_t)v + 2]);
+                    }
+                    l[q] = loc[v];
+                }
+                cell_ln[p] = make_uint2(l[0] | l[1] << 16, l[2] | l[3] << 16);
+            }
+            grp_off.push_back(grp_off.back() + cnt);
+        }
+    }
     ctx->Ncell = (uint32_t)cell0;
     ctx->nbf = (uint32_t)bface.size();
 
@@ -802,6 +833,11 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     if ((rc = dalloc(ctx, &ctx->cnodes, cnodes.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->nbr, nbr.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->fcode, fcode.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->grp_off, grp_off.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->gx, gx.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->gy, gy.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->gz, gz.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->cell_ln, cell_ln.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->blk_inst, blk_inst.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->inst_end, inst_end.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->st, (size_t)n))) return rc;
@@ -832,6 +868,11 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     HIPC(hipMemcpy(ctx->cnodes, cnodes.data(), cnodes.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->fcode, fcode.data(), fcode.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->grp_off, grp_off.data(), grp_off.size() * 4, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->gx, gx.data(), gx.size() * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->gy, gy.data(), gy.size() * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->gz, gz.data(), gz.size() * 8, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(ctx->cell_ln, cell_ln.data(), cell_ln.size() * sizeof(uint2), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->blk_inst, blk_inst.data(), blk_inst.size() * 4, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->inst_end, inst_end.data(), inst_end.size() * 4, hipMemcpyHostToDevice));
     if (ctx->nbf) {
