@@ -42,7 +42,7 @@ namespace nf {
 #define NF_A_PREFETCH 0
 #endif
 #ifndef NF_B_GROUP
-#define NF_B_GROUP 2
+#define NF_B_GROUP 1
 #endif
 // waves per SIMD the sweeps are compiled for (0: the compiler's choice)
 #ifndef NF_SWEEP_WAVES
@@ -52,6 +52,14 @@ namespace nf {
 #define NF_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(NF_SWEEP_WAVES, NF_SWEEP_WAVES)))
 #else
 #define NF_SWEEP_ATTR
+#endif
+#ifndef NF_SWEEPB_WAVES
+#define NF_SWEEPB_WAVES 0
+#endif
+#if NF_SWEEPB_WAVES
+#define NF_SWEEPB_ATTR __attribute__((amdgpu_waves_per_eu(NF_SWEEPB_WAVES, NF_SWEEPB_WAVES)))
+#else
+#define NF_SWEEPB_ATTR NF_SWEEP_ATTR
 #endif
 
 constexpr int BLOCK = 256;  // threads of the auxiliary kernels; instance padding unit
@@ -452,7 +460,7 @@ struct GhostArgs {
     const uint16_t *binst;       // instance
     uint32_t nb;                 // boundary faces
     const double *src;           // V [tile][5][64] (cell) or FV [tile][4][5][64] (face)
-    double *out;                 // [nb][5]
+    double *out;                 // [nb/64][5][64], tiled like V
     const FarField<double> *fs;  // [instances]
 };
 
@@ -478,14 +486,14 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
     ghost_d((int)a.bbc[b], Vd, S, a.fs[inst], Vg);
     if (live && !m.st[inst].stop) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+        for (int k = 0; k < 5; ++k) a.out[tix(b, k)] = Vg[k];
     }
 }
 
 // ---------------------------------------------------------------------------
 struct SweepAArgs {
     const double *V;   // [tile][5][64]
-    const double *Gc;  // [nb][5] ghost of the cell-centre state
+    const double *Gc;  // [nb/64][5][64] ghost of the cell-centre state
     double *FV;        // [tile][4][5][64]
     double *dt_part;   // per-block CFL dt minimum, or null
     double *dt_cell;   // [N] per-cell CFL step (local time stepping), or null
@@ -514,23 +522,22 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
     double Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
-    // neighbour (or ghost) states; NF_A_PREFETCH = 1 loads variable k + 1
-    // while variable k is processed instead of all five up front, which
-    // trades memory-level parallelism for registers
+    // neighbour (or ghost) states; ghosts are tiled like V, so every source
+    // has the same stride and the five loads share one address.
+    // NF_A_PREFETCH = 1 loads variable k + 1 while variable k is processed
+    // instead of all five up front (fewer registers, less parallelism)
     double Vn[4][5];
     const double *nsrc[4];
-    uint32_t nstride[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t idx = w[s] & 0x1FFFFFFFu;
-        nsrc[s] = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
-        nstride[s] = (w[s] & 0x80000000u) ? 1 : TILE;
+        nsrc[s] = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
     }
     if (ORDER != 0) {
 #pragma unroll
         for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) Vn[s][k] = nsrc[s][k * nstride[s]];
+            for (int s = 0; s < 4; ++s) Vn[s][k] = nsrc[s][k * TILE];
     }
     CellGeo g;
     const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
@@ -554,7 +561,7 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
         for (int k = 0; k < 5; ++k) {
             if (NF_A_PREFETCH && k + 1 < 5) {
 #pragma unroll
-                for (int s = 0; s < 4; ++s) Vn[s][k + 1] = nsrc[s][(k + 1) * nstride[s]];
+                for (int s = 0; s < 4; ++s) Vn[s][k + 1] = nsrc[s][(k + 1) * TILE];
             }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
@@ -608,7 +615,7 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 // ---------------------------------------------------------------------------
 struct SweepBArgs {
     const double *FV;       // [tile][4][5][64]
-    const double *Gf;       // [nb][5] ghost of the reconstructed face state
+    const double *Gf;       // [nb/64][5][64] ghost of the reconstructed face state
     const double *U;        // [tile][5][64] state entering the stage
     const double *Uo;       // U_old (U^n)
     double *Uout;           // (may alias U)
@@ -622,7 +629,7 @@ struct SweepBArgs {
 };
 
 template <int B>
-__global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArgs a) {
+__global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
@@ -634,8 +641,9 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArg
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    // face values of both sides, loaded NF_B_GROUP faces at a time (2, the
-    // default, measured 4% faster than issuing all four faces' loads up front)
+    // face values of both sides, loaded NF_B_GROUP faces at a time; one face
+    // at a time measured fastest at 128^3 (1.11 ms vs 1.14 for two faces and
+    // 1.20 for all four faces' loads issued up front)
     const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
     __builtin_amdgcn_sched_barrier(0);
     CellGeo g;
@@ -649,12 +657,11 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepB(DevMesh m, SweepBArg
             const int s = g0 + q;
             const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
             const bool bnd = w[s] & 0x80000000u;
-            const double *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
-            const size_t stride = bnd ? 1 : TILE;
+            const double *src = bnd ? a.Gf + tix(idx, 0) : a.FV + fix(idx, sn, 0);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 Vo4[q][k] = a.FV[fix(c, s, k)];
-                Vn4[q][k] = src[k * stride];
+                Vn4[q][k] = src[k * TILE];
             }
         }
         if (g0 == 0) {

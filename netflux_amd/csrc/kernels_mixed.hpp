@@ -22,7 +22,7 @@ struct GhostArgsMx {
     const uint16_t *binst;
     uint32_t nb;
     const float *src;          // V32 [tile][5][64] (cell) or FV32 (face)
-    float *out;                // [nb][5]
+    float *out;                // [nb/64][5][64], tiled like V
     const FarField<float> *fs; // [instances]
 };
 
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_ghost_mx(DevMesh m, GhostArgsMx a) {
     ghost_value<float>((int)a.bbc[b], Vd, S, a.fs[inst], Vg);
     if (live && !m.st[inst].stop) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) a.out[5 * (size_t)b + k] = Vg[k];
+        for (int k = 0; k < 5; ++k) a.out[tix(b, k)] = Vg[k];
     }
 }
 
@@ -168,10 +168,9 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const uint32_t idx = w[s] & 0x1FFFFFFFu;
-            const float *src = (w[s] & 0x80000000u) ? a.Gc + 5 * (size_t)idx : a.V + tix(idx, 0);
-            const size_t stride = (w[s] & 0x80000000u) ? 1 : TILE;
+            const float *src = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * stride];
+            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * TILE];
         }
     }
     CellGeo g;
@@ -256,12 +255,11 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
     for (int s = 0; s < 4; ++s) {
         const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
         const bool bnd = w[s] & 0x80000000u;
-        const float *src = bnd ? a.Gf + 5 * (size_t)idx : a.FV + fix(idx, sn, 0);
-        const size_t stride = bnd ? 1 : TILE;
+        const float *src = bnd ? a.Gf + tix(idx, 0) : a.FV + fix(idx, sn, 0);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             Vo4[s][k] = a.FV[fix(c, s, k)];
-            Vn4[s][k] = src[k * stride];
+            Vn4[s][k] = src[k * TILE];
         }
     }
     CellGeo g;

@@ -846,13 +846,13 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     if (ctx->mixed) {
         if ((rc = dalloc(ctx, &ctx->V32, 5 * NC))) return rc;
         if ((rc = dalloc(ctx, &ctx->FV32, 20 * NC))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
     } else {
         if ((rc = dalloc(ctx, &ctx->V, 5 * NC))) return rc;
         if ((rc = dalloc(ctx, &ctx->FV, 20 * NC))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
     }
     if (p0.local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NC))) return rc;
     if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
@@ -1347,14 +1347,14 @@ int nf_set_precision(nf_ctx *ctx, int32_t precision) {
     if (mixed && !ctx->V32) {
         if ((rc = dalloc(ctx, &ctx->V32, 5 * NC))) return rc;
         if ((rc = dalloc(ctx, &ctx->FV32, 20 * NC))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc32, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf32, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
     }
     if (!mixed && !ctx->V) {
         if ((rc = dalloc(ctx, &ctx->V, 5 * NC))) return rc;
         if ((rc = dalloc(ctx, &ctx->FV, 20 * NC))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gc, 5 * (size_t)ctx->nbf))) return rc;
-        if ((rc = dalloc(ctx, &ctx->Gf, 5 * (size_t)ctx->nbf))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gc, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
+        if ((rc = dalloc(ctx, &ctx->Gf, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
     }
     HIPC(hipStreamSynchronize(ctx->stream));
     ctx->mixed = mixed;
