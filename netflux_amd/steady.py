@@ -114,6 +114,10 @@ class BatchRun:
         self.cfgs = [config_of(c, precision, local_time_step, cfl, limiter) for c in cases]
         self.batch = S.BatchSolver(self.cfgs, self.meshes)
         self.precision = precision
+        self.keep_switch = False  # keep the states reached when mixed precision hands over to FP64
+        self.switch_solutions = None
+        self.switch_steps = None
+        self.switch_drops = None
         self.tune = None
         self.autotune_steps = autotune_steps
 
@@ -160,10 +164,10 @@ class BatchRun:
                     s0["res_last"] = s1["res_last"] if s1["steps_done"] else s0["res_last"]
             done += k
             n_conv = sum(s["converged"] for s in st)
-            # an unconverged instance whose residual fell by less than 30% over
+            # an unconverged instance whose residual fell by less than half over
             # the chunk has reached the noise floor of the current precision
             stalled = sum(1 for s, p in zip(st, part) if not s["converged"] and p["steps_done"] and
-                          p["res_last"] > 0.7 * s.get("res_prev", np.inf))
+                          p["res_last"] > 0.5 * s.get("res_prev", np.inf))
             for s, p in zip(st, part):
                 if p["steps_done"]:
                     s["res_prev"] = p["res_last"]
@@ -172,6 +176,9 @@ class BatchRun:
             if n_conv == b.n or (phase + 1 < len(phases) and n_conv + stalled == b.n):
                 if phase + 1 == len(phases):
                     break
+                self.switch_steps = done
+                self.switch_drops = [s["res_last"] / r for s, r in zip(st, r_fs)]
+                self.switch_solutions = self.solutions() if self.keep_switch else None
                 phase += 1
                 b.set_precision(phases[phase][0])
                 _log(f"[solve] switching to {phases[phase][0]} after {done} steps")
@@ -288,6 +295,12 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
         return {"wall_p_maxabs_max": max(dp), "wall_p_maxabs_median": float(np.median(dp)),
                 "U_maxabs_max": max(du), "U_relL2_max": max(dl)}
 
+    run64 = BatchRun(cases, precision="fp64", autotune_steps=20)
+    st, wall = run64.solve(drop, max_steps, check_every)
+    _log(f"[batch fp64] {_summary(st)} {wall:.2f} s, tune {run64.tune}")
+    rep["batch_fp64"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases, "tune": run64.tune,
+                                        "steps": [s["steps_done"] for s in st]} | compare(run64.solutions())
+    save()
     run = BatchRun(cases, precision="mixed", autotune_steps=20)
     st, wall = run.solve(drop, max_steps, check_every)
     _log(f"[batch mixed] {_summary(st)} {wall:.2f} s, tune {run.tune}")
@@ -312,6 +325,44 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
         rep["batch_mixed_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases,
                                                    "steps": [s["steps_done"] for s in st]} | compare(run.solutions())
         save()
+        st, wall = run64.solve(drop, max_steps, check_every, net=net, stall_window=4000, stall_ratio=0.5)
+        _log(f"[batch fp64 warm] {_summary(st)} {wall:.2f} s")
+        rep["batch_fp64_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases,
+                                                  "steps": [s["steps_done"] for s in st]} | compare(run64.solutions())
+        save()
+    return rep
+
+
+def c3(drop=1e-8, max_steps=500000, check_every=200):
+    """The 2M-tet channel (160 x 64 x 32, Mach 0.5, h = 0.05): FP64 with the
+    reference's global time step against mixed precision with local steps --
+    at its FP32 noise floor and after the FP64 finish -- at steady state."""
+    case = Case(160, 64, 32, 0.5, 0.05)
+    rep = {"case": case.__dict__ | {"cells": case.n_cells}, "drop": drop}
+    ref = BatchRun([case], precision="fp64", local_time_step=False)
+    st, wall = ref.solve(drop, max_steps, check_every)
+    U64 = ref.solutions()[0]
+    rep["fp64_global_dt"] = _summary(st) | {"wall_s": wall}
+    _log(f"[c3 fp64] {rep['fp64_global_dt']}")
+    del ref
+    mx = BatchRun([case], precision="mixed")
+    mx.keep_switch = True
+    st, wall = mx.solve(drop, max_steps, check_every)
+    rep["mixed_then_fp64"] = _summary(st) | {"wall_s": wall, "switch_steps": mx.switch_steps,
+                                             "drop_at_switch": mx.switch_drops[0]}
+    _log(f"[c3 mixed] {rep['mixed_then_fp64']}")
+    mesh = mx.meshes[0]
+    _, p64 = wall_pressure(mesh, U64)
+
+    def err(U):
+        _, p = wall_pressure(mesh, U)
+        return {"U_maxabs": float(np.max(np.abs(U - U64))),
+                "U_relL2": float(np.linalg.norm(U - U64) / np.linalg.norm(U64)),
+                "wall_p_maxabs": float(np.max(np.abs(p - p64))),
+                "wall_p_relL2": float(np.linalg.norm(p - p64) / np.linalg.norm(p64))}
+
+    rep["error_mixed_at_floor"] = err(mx.switch_solutions[0])
+    rep["error_mixed_then_fp64"] = err(mx.solutions()[0])
     return rep
 
 
@@ -333,7 +384,11 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-lts", action="store_true")
     ap.add_argument("--partial", default=None, help="write the report after each phase to this file")
+    ap.add_argument("--c3", action="store_true", help="the 2M-tet mixed-precision channel case only")
     a = ap.parse_args(argv)
+    if a.c3:
+        print(json.dumps(c3(a.drop, a.max_steps, a.check_every)))
+        return
     rep = study(a.cases, a.seed, a.cells_lo, a.cells_hi, a.drop, a.max_steps, a.check_every, a.train_cases,
                 a.train_cells, a.per_case, a.epochs, plain=not a.no_plain, plain_lts=a.plain_lts,
                 partial=a.partial)
