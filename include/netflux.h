@@ -96,6 +96,20 @@ int nf_mesh_box(uint32_t nx, uint32_t ny, uint32_t nz,
 int nf_mesh_from_desc(const nf_mesh_desc *desc, nf_mesh **out);
 int nf_mesh_get_desc(const nf_mesh *mesh, nf_mesh_desc *desc);
 void nf_mesh_destroy(nf_mesh *mesh);
+/* SU2 mesh file (PrimalGrid::read_su2_mesh, src/geometry/PrimalGrid.cpp:
+ * 140-244): tetrahedra (type 10, Salome node order, stored in VTK order by
+ * swapping nodes 1 and 2 as src/geometry/GeometryUtilities.cpp:443-466
+ * does) and triangle markers (type 5) only.  Marker i takes the boundary
+ * type bcs[j] of names[j] == its MARKER_TAG; a marker the list does not
+ * name is an error, as in the reference.  Patch order = marker order. */
+int nf_mesh_read_su2(const char *path, int32_t n_names, const char *const *names, const int32_t *bcs,
+                     nf_mesh **out);
+/* Writes tetrahedra in Salome order and one marker per patch, coordinates
+ * with 17 significant digits; marker tags from names (or as read, or
+ * "patch_<i>"). */
+int nf_mesh_write_su2(const nf_mesh *mesh, const char *path, int32_t n_names, const char *const *names);
+int nf_mesh_patch_name(const nf_mesh *mesh, int32_t patch, char *buf, int32_t len);
+const char *nf_mesh_last_error(void);   /* text of the last NF_E_MESH of these calls */
 
 /* The finite-volume face graph built exactly as FV_Grid does
  * (src/geometry/FV_Grid.cpp:21-267): interior faces sorted by (i,j), i<j,

@@ -93,6 +93,10 @@ SIGNATURES = [
     ("nf_mesh_from_desc", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("nf_mesh_get_desc", C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
     ("nf_mesh_destroy", None, [C.c_void_p]),
+    ("nf_mesh_read_su2", C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), i32p, C.POINTER(C.c_void_p)]),
+    ("nf_mesh_write_su2", C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_char_p)]),
+    ("nf_mesh_patch_name", C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
+    ("nf_mesh_last_error", C.c_char_p, []),
     ("nf_fv_sizes_of", C.c_int, [C.c_void_p, C.POINTER(FVSizes)]),
     ("nf_fv_export", C.c_int, [C.c_void_p, u32p, f64p, f64p, f64p, f64p, f64p, u32p, u32p, f64p]),
     ("nf_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]),

@@ -12,6 +12,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace nf {
@@ -66,7 +67,12 @@ constexpr uint32_t FC_OWNER = 0x80u;
 
 struct MeshError {
     std::string msg;
+    MeshError(std::string m = std::string()) : msg(std::move(m)) {}
 };
+
+// SU2 files (su2.cpp): tetrahedra + triangle markers; marker names in file order
+void read_su2(const std::string &path, PrimalMesh &m, std::vector<std::string> &names);
+void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<std::string> &names);
 
 void build_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly,
                double lz, double jitter, uint64_t seed, double bump_h,

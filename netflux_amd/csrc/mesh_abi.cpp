@@ -1,7 +1,10 @@
 // C ABI of the host mesh utilities (include/netflux.h): generators and the
 // finite-volume face graph export.  CPU only.
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "../../include/netflux.h"
 #include "mesh.hpp"
@@ -10,9 +13,12 @@ struct nf_mesh {
     nf::PrimalMesh pm;
     nf::FVGrid fv;
     bool fv_built = false;
+    std::vector<std::string> names;  // patch names (SU2 marker tags)
 };
 
 namespace {
+thread_local std::string g_mesh_err;
+
 int ensure_fv(nf_mesh *m) {
     if (m->fv_built) return NF_OK;
     try {
@@ -78,6 +84,60 @@ int nf_mesh_get_desc(const nf_mesh *m, nf_mesh_desc *d) {
 }
 
 void nf_mesh_destroy(nf_mesh *m) { delete m; }
+
+const char *nf_mesh_last_error(void) { return g_mesh_err.c_str(); }
+
+int nf_mesh_read_su2(const char *path, int32_t n_names, const char *const *names, const int32_t *bcs,
+                     nf_mesh **out) {
+    if (!path || !out || n_names < 0 || (n_names > 0 && (!names || !bcs))) return NF_E_ARG;
+    *out = nullptr;
+    nf_mesh *m = new (std::nothrow) nf_mesh;
+    if (!m) return NF_E_ARG;
+    try {
+        nf::read_su2(path, m->pm, m->names);
+        // boundary type of each marker from the patch list of the input file
+        // (PrimalGrid::read_su2_mesh rejects a marker the input does not name)
+        for (const std::string &name : m->names) {
+            int32_t bc = -1;
+            for (int32_t i = 0; i < n_names; ++i)
+                if (name == names[i]) bc = bcs[i];
+            if (bc < 0) throw nf::MeshError("Patch with name '" + name + "' is not named in input file");
+            if (bc > 2) throw nf::MeshError("bad boundary type for patch '" + name + "'");
+            m->pm.patch_bc.push_back(bc);
+        }
+    } catch (const nf::MeshError &e) {
+        g_mesh_err = e.msg;
+        delete m;
+        return NF_E_MESH;
+    }
+    *out = m;
+    return NF_OK;
+}
+
+int nf_mesh_write_su2(const nf_mesh *m, const char *path, int32_t n_names, const char *const *names) {
+    if (!m || !path || n_names < 0 || (n_names > 0 && !names)) return NF_E_ARG;
+    std::vector<std::string> tags;
+    for (size_t q = 0; q < m->pm.patch_n.size(); ++q) {
+        if ((int32_t)q < n_names && names[q]) tags.push_back(names[q]);
+        else if (q < m->names.size()) tags.push_back(m->names[q]);
+        else tags.push_back("patch_" + std::to_string(q));
+        if (tags.back().empty() || tags.back().find_first_of(" \t\n") != std::string::npos) return NF_E_ARG;
+    }
+    try {
+        nf::write_su2(path, m->pm, tags);
+    } catch (const nf::MeshError &e) {
+        g_mesh_err = e.msg;
+        return NF_E_MESH;
+    }
+    return NF_OK;
+}
+
+int nf_mesh_patch_name(const nf_mesh *m, int32_t i, char *buf, int32_t len) {
+    if (!m || !buf || len < 1 || i < 0 || i >= (int32_t)m->pm.patch_n.size()) return NF_E_ARG;
+    const std::string s = i < (int32_t)m->names.size() ? m->names[i] : "patch_" + std::to_string(i);
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
+    return NF_OK;
+}
 
 int nf_fv_sizes_of(const nf_mesh *mc, nf_fv_sizes *s) {
     if (!mc || !s) return NF_E_ARG;

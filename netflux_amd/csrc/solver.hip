@@ -1225,10 +1225,12 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
     if ((rc = prim_all(ctx))) return rc;
     if ((rc = run_sweepA(ctx, false, true))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, ctx->Ub, false, true))) return rc;
-    // the sweep wrote prim of the discarded update: primvars = prim(U) again
+    // the sweep wrote prim of the discarded update: primvars = prim(U) again,
+    // which is what they were before the call.  The dt the last step left on
+    // the device (from prim(U2) for TVD-RK3) stays valid, so an evaluation
+    // between steps does not change the run.
     if ((rc = prim_all(ctx))) return rc;
     ctx->aux_valid = true;
-    ctx->inst[0].dt_valid = false;
     return check_errors(ctx);
 }
 

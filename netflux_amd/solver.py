@@ -115,6 +115,36 @@ class Mesh:
                        patches=("FarField", "FarField") + ("SlipWall",) * 4)
 
     @classmethod
+    def read_su2(cls, path, patch_bc):
+        """SU2 file with tetrahedra and triangle markers; patch_bc maps each
+        marker tag to a boundary type (name or NF_BC_* value)."""
+        names = list(patch_bc.keys())
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        bcs = np.array([BC_NAMES[b] if isinstance(b, str) else int(b) for b in patch_bc.values()] or [0],
+                       dtype=np.int32)
+        h = C.c_void_p()
+        rc = A.lib().nf_mesh_read_su2(str(path).encode(), len(names), arr, _i32(bcs), C.byref(h))
+        if rc != A.NF_OK:
+            raise RuntimeError(f"Error reading su2 mesh '{path}': {A.lib().nf_mesh_last_error().decode()}")
+        return cls(h)
+
+    def write_su2(self, path, names=None):
+        names = list(names or [])
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        rc = A.lib().nf_mesh_write_su2(self._h, str(path).encode(), len(names), arr)
+        if rc != A.NF_OK:
+            raise RuntimeError(f"nf_mesh_write_su2 failed ({rc}): {A.lib().nf_mesh_last_error().decode()}")
+
+    @property
+    def patch_names(self):
+        out = []
+        buf = C.create_string_buffer(256)
+        for i in range(self._desc.n_patches):
+            _check(A.lib().nf_mesh_patch_name(self._h, i, buf, 256), what="nf_mesh_patch_name")
+            out.append(buf.value.decode())
+        return out
+
+    @classmethod
     def from_arrays(cls, nodes, cells, patch_faces, patch_bc):
         """nodes [n][3] f64, cells [N][4] u32 (VTK order), patch_faces list of [m][3]."""
         nodes = np.ascontiguousarray(nodes, dtype=np.float64)

@@ -158,3 +158,21 @@ def test_residual_history_matches_oracle():
         expect.append(np.linalg.norm(ora.get("R")[:, 0]))
         ora.step(dt)
     np.testing.assert_allclose(res, expect, rtol=1e-13)
+
+
+@pytest.mark.parametrize("scheme", ["TVD_RK3", "ExplicitEuler"])
+def test_flux_balance_between_steps_leaves_run_unchanged(scheme):
+    """Evaluating the flux balance between steps (debug output) keeps the
+    primvars and the device dt the last step left, so the trajectory is
+    the oracle's."""
+    mesh = nf.Mesh.box(6, jitter=0.1, seed=9)
+    cfg = sod_config(time_scheme=scheme)
+    gpu, ora = _pair(mesh, cfg, sod_state(mesh))
+    for n in range(6):
+        gpu.eval_flux_balance()
+        gpu.calc_timestep(cfg)
+        dt = ora.calc_timestep()
+        assert cfg.delta_time == dt, f"dt differs at step {n}"
+        gpu.step(cfg)
+        ora.step(dt)
+    np.testing.assert_array_equal(gpu.get_solver_data().get_solution(), ora.get("U"))
