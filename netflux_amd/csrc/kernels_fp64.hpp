@@ -44,6 +44,10 @@ namespace nf {
 #ifndef NF_B_GROUP
 #define NF_B_GROUP 1
 #endif
+// sweep A reads neighbours in the lane's own 64-cell tile from LDS
+#ifndef NF_A_LDS
+#define NF_A_LDS 1
+#endif
 // waves per SIMD the sweeps are compiled for (0: the compiler's choice)
 #ifndef NF_SWEEP_WAVES
 #define NF_SWEEP_WAVES 0
@@ -64,6 +68,12 @@ namespace nf {
 
 constexpr int BLOCK = 256;  // threads of the auxiliary kernels; instance padding unit
 constexpr int TILE = 64;    // cells per layout tile
+
+// load through a global-address-space pointer (keeps the compiler from
+// falling back to flat loads, which also count against the LDS counter)
+__device__ __forceinline__ double ldg(const double *p) {
+    return *(const __attribute__((address_space(1))) double *)p;
+}
 
 __device__ __forceinline__ size_t tix(uint32_t c, int k) {  // [tile][5][64]
     return ((size_t)(c >> 6) * 5 + k) * TILE + (c & 63u);
@@ -243,13 +253,27 @@ __device__ __forceinline__ void ghost_d(int bc, const double *Vd, V3d S, const F
     }
 }
 
-// Barth face contribution (include/Numerics.hpp:190-206); min(1, x) as
-// minNum, which also returns 1 for a NaN quotient like std::min(1.0, x)
-__device__ __forceinline__ double barth_phi_d(double delta, double vmax, double vmin, double v) {
-    const double d2 = double(isign(delta)) * (fabs(delta) + DBL_EPSILON);
-    const double num = (d2 > 0.0) ? vmax - v : vmin - v;  // one divide for both branches
-    const double q = num / d2;
-    return (d2 != 0.0) ? dmin(1.0, q) : 1.0;
+// Barth limiter of one variable over the cell's four faces
+// (include/Numerics.hpp:179-236) with two divisions instead of four.  Faces
+// with Delta_2 > 0 all divide vmax - v >= 0 by a positive Delta_2, so the
+// smallest quotient is the one with the largest Delta_2, and because
+// correctly rounded division is monotone, min_s round(A / d_s) =
+// round(A / max_s d_s); likewise vmin - v <= 0 over the most negative
+// Delta_2.  min(1, .) and the minimum over faces commute with this, a NaN
+// Delta_2 falls in neither group (contribution 1, as in the reference) and
+// Delta_2 == 0 contributes 1.  Only the sign of a zero limiter can differ
+// from a face-by-face minimum, and it cannot reach a nonzero result.
+__device__ __forceinline__ double barth_psi_d(const double *delta, double vmax, double vmin, double v) {
+    double dp = 0.0, dm = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const double d2 = double(isign(delta[s])) * (fabs(delta[s]) + DBL_EPSILON);
+        dp = dmax(dp, d2);
+        dm = dmin(dm, d2);
+    }
+    const double pp = (dp > 0.0) ? dmin(1.0, (vmax - v) / dp) : 1.0;
+    const double pm = (dm < 0.0) ? dmin(1.0, (vmin - v) / dm) : 1.0;
+    return dmin(pp, pm);
 }
 
 // Logical block index: blocks are dispatched round robin over the 8 XCDs;
@@ -508,44 +532,74 @@ template <int ORDER, int B>
 __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
+    __shared__ double ldsv[NF_A_LDS ? 5 * B : 1];  // [wave][5][64] own states
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
     if (m.st[L.inst].stop) return;
     const uint32_t c = L.c;
     const bool live = L.live;
+    const uint32_t p = L.lb * B + threadIdx.x;  // device position (padding included)
 
     // all loads first: own state, neighbour states (or ghost states), nodes
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
+    const NodeTableLoad ntl = node_table_issue(m, p);
     __builtin_amdgcn_sched_barrier(0);
     double Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
     // neighbour (or ghost) states; ghosts are tiled like V, so every source
-    // has the same stride and the five loads share one address.
+    // has the same stride and the five loads share one address.  A neighbour
+    // in the lane's own tile (most of them in Morton order) is read from the
+    // wave's copy of the tile in LDS instead (NF_A_LDS): the gathers are
+    // exec-masked to the other lanes, which is what saves time (the
+    // texture path's work grows with the active lanes).  Every cell of the
+    // tile is a lane of this wave and live, since neighbours are real cells.
+    // The gathers go first: the waits for the node and state loads then
+    // cover them too (masked loads cannot be counted), which measured
+    // faster than issuing them after the LDS copies (1.04 vs 0.93 ms).
     // NF_A_PREFETCH = 1 loads variable k + 1 while variable k is processed
     // instead of all five up front (fewer registers, less parallelism)
     double Vn[4][5];
     const double *nsrc[4];
+    bool loc[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t idx = w[s] & 0x1FFFFFFFu;
         nsrc[s] = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
+        loc[s] = NF_A_LDS && !(w[s] & 0x80000000u) && (idx >> 6) == (p >> 6);
     }
     if (ORDER != 0) {
 #pragma unroll
-        for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k)
+        for (int s = 0; s < 4; ++s)
+            if (!loc[s]) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) Vn[s][k] = nsrc[s][k * TILE];
+                for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k) Vn[s][k] = ldg(nsrc[s] + k * TILE);
+            }
+    }
+    double *vt = ldsv + (NF_A_LDS ? (threadIdx.x >> 6) * 5 * TILE : 0);
+    if (NF_A_LDS && ORDER != 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vt[k * TILE + (p & 63u)] = Vc[k];
     }
     CellGeo g;
-    const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
+    // (its wave-level fences also order the state writes above before the
+    // reads below)
+    const NodeView nv = node_table_finish<B>(m, p, ntl, lds, g);
     cell_geo(m, g);
     V3d S[4], r[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) face_geo(m, g, nv, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     __builtin_amdgcn_sched_barrier(0);
+    if (NF_A_LDS && ORDER != 0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (loc[s]) {
+                const uint32_t j = w[s] & 63u;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) Vn[s][k] = vt[k * TILE + j];
+            }
+    }
 
     if (ORDER == 0) {
         if (live) {
@@ -561,12 +615,13 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
         for (int k = 0; k < 5; ++k) {
             if (NF_A_PREFETCH && k + 1 < 5) {
 #pragma unroll
-                for (int s = 0; s < 4; ++s) Vn[s][k + 1] = nsrc[s][(k + 1) * TILE];
+                for (int s = 0; s < 4; ++s)
+                    if (!loc[s]) Vn[s][k + 1] = nsrc[s][(k + 1) * TILE];
             }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
             double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            double vmax = -DBL_MAX, vmin = DBL_MAX;
+            double vmax = Vc[k], vmin = Vc[k];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 // 0.5 (V_i + V_j) is symmetric in i, j; the non-owner subtracts
@@ -576,19 +631,16 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
                 g1 = g1 + flip(rdiv(uf * S[s].y, g.ivol), mneg);
                 g2 = g2 + flip(rdiv(uf * S[s].z, g.ivol), mneg);
                 if (ORDER == 2) {
-                    // neighbour min/max (include/Numerics.hpp:276-291)
-                    vmax = dmax(vmax, dmax(Vc[k], Vn[s][k]));
-                    vmin = dmin(vmin, dmin(Vc[k], Vn[s][k]));
+                    // neighbour min/max (include/Numerics.hpp:276-291): the
+                    // maximum of {v, v_j} over the faces, one v_j at a time
+                    vmax = dmax(vmax, Vn[s][k]);
+                    vmin = dmin(vmin, Vn[s][k]);
                 }
             }
             double delta[4];
-            double psi = (ORDER == 2) ? DBL_MAX : 1.0;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                delta[s] = (g0 * r[s].x + g1 * r[s].y) + g2 * r[s].z;
-                // Barth limiter (include/Numerics.hpp:179-236), faces in order
-                if (ORDER == 2) psi = dmin(psi, barth_phi_d(delta[s], vmax, vmin, Vc[k]));
-            }
+            for (int s = 0; s < 4; ++s) delta[s] = (g0 * r[s].x + g1 * r[s].y) + g2 * r[s].z;
+            const double psi = (ORDER == 2) ? barth_psi_d(delta, vmax, vmin, Vc[k]) : 1.0;
             if (live) {
                 // limited reconstruction (include/Numerics.hpp:135-145)
 #pragma unroll
@@ -641,14 +693,14 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    // face values of both sides, loaded NF_B_GROUP faces at a time; one face
-    // at a time measured fastest at 128^3 (1.11 ms vs 1.14 for two faces and
-    // 1.20 for all four faces' loads issued up front)
     const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
     __builtin_amdgcn_sched_barrier(0);
     CellGeo g;
     NodeView nv;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // face values of both sides, loaded NF_B_GROUP faces at a time; one face
+    // at a time measured fastest at 128^3 (1.11 ms vs 1.14 for two faces and
+    // 1.20 for all four faces' loads issued up front)
 #pragma unroll
     for (int g0 = 0; g0 < 4; g0 += NF_B_GROUP) {
         double Vo4[NF_B_GROUP][5], Vn4[NF_B_GROUP][5];
