@@ -48,6 +48,10 @@ namespace nf {
 #ifndef NF_A_LDS
 #define NF_A_LDS 1
 #endif
+// sweep A writes its face values with 16-byte stores (needs NF_A_LDS)
+#ifndef NF_A_STX4
+#define NF_A_STX4 1
+#endif
 // waves per SIMD the sweeps are compiled for (0: the compiler's choice)
 #ifndef NF_SWEEP_WAVES
 #define NF_SWEEP_WAVES 0
@@ -263,16 +267,19 @@ __device__ __forceinline__ void ghost_d(int bc, const double *Vd, V3d S, const F
 // Delta_2 falls in neither group (contribution 1, as in the reference) and
 // Delta_2 == 0 contributes 1.  Only the sign of a zero limiter can differ
 // from a face-by-face minimum, and it cannot reach a nonzero result.
+// Delta_2 = sign(d) (|d| + eps) is positive exactly when d is, and rounding
+// of x + eps is monotone in x, so the largest positive Delta_2 is
+// max(d) + eps and the most negative is min(d) - eps (= -(|min d| + eps));
+// maxNum/minNum against 0 drop NaN and the values of the other sign.
 __device__ __forceinline__ double barth_psi_d(const double *delta, double vmax, double vmin, double v) {
-    double dp = 0.0, dm = 0.0;
+    double hi = 0.0, lo = 0.0;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const double d2 = double(isign(delta[s])) * (fabs(delta[s]) + DBL_EPSILON);
-        dp = dmax(dp, d2);
-        dm = dmin(dm, d2);
+        hi = dmax(hi, delta[s]);
+        lo = dmin(lo, delta[s]);
     }
-    const double pp = (dp > 0.0) ? dmin(1.0, (vmax - v) / dp) : 1.0;
-    const double pm = (dm < 0.0) ? dmin(1.0, (vmin - v) / dm) : 1.0;
+    const double pp = (hi > 0.0) ? dmin(1.0, (vmax - v) / (hi + DBL_EPSILON)) : 1.0;
+    const double pm = (lo < 0.0) ? dmin(1.0, (vmin - v) / (lo - DBL_EPSILON)) : 1.0;
     return dmin(pp, pm);
 }
 
@@ -532,7 +539,7 @@ template <int ORDER, int B>
 __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
-    __shared__ double ldsv[NF_A_LDS ? 5 * B : 1];  // [wave][5][64] own states
+    __shared__ __align__(16) double ldsv[NF_A_LDS ? 5 * B : 2];  // [wave][5][64] own states
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
     if (m.st[L.inst].stop) return;
     const uint32_t c = L.c;
@@ -609,6 +616,14 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
                 for (int k = 0; k < 5; ++k) a.FV[fix(c, s, k)] = Vc[k];
         }
     } else {
+        // face normals as the cell adds them: -S where the cell is the
+        // reference's neighbour j (cell j subtracts tmp/vol)
+        V3d Sg[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
+            Sg[s] = {flip(S[s].x, mneg), flip(S[s].y, mneg), flip(S[s].z, mneg)};
+        }
         // One variable at a time: gradient row k, its neighbour min/max, the
         // Barth factor and the face values depend on variable k only.
 #pragma unroll
@@ -625,11 +640,12 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 // 0.5 (V_i + V_j) is symmetric in i, j; the non-owner subtracts
-                const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
+                // (through Sg, whose sign flip carries through the product
+                // and the division exactly)
                 const double uf = 0.5 * (Vc[k] + Vn[s][k]);
-                g0 = g0 + flip(rdiv(uf * S[s].x, g.ivol), mneg);
-                g1 = g1 + flip(rdiv(uf * S[s].y, g.ivol), mneg);
-                g2 = g2 + flip(rdiv(uf * S[s].z, g.ivol), mneg);
+                g0 = g0 + rdiv(uf * Sg[s].x, g.ivol);
+                g1 = g1 + rdiv(uf * Sg[s].y, g.ivol);
+                g2 = g2 + rdiv(uf * Sg[s].z, g.ivol);
                 if (ORDER == 2) {
                     // neighbour min/max (include/Numerics.hpp:276-291): the
                     // maximum of {v, v_j} over the faces, one v_j at a time
@@ -641,10 +657,32 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 #pragma unroll
             for (int s = 0; s < 4; ++s) delta[s] = (g0 * r[s].x + g1 * r[s].y) + g2 * r[s].z;
             const double psi = (ORDER == 2) ? barth_psi_d(delta, vmax, vmin, Vc[k]) : 1.0;
-            if (live) {
-                // limited reconstruction (include/Numerics.hpp:135-145)
+            // limited reconstruction (include/Numerics.hpp:135-145)
+            if (NF_A_LDS && NF_A_STX4) {
+                // The wave's four rows (s, k) of the tile are written with
+                // 16-byte stores: the values pass through the LDS slice (free
+                // once the neighbour states are read), and lane l stores the
+                // values of lanes 2(l%32), 2(l%32)+1 of slots l/32 and l/32+2.
+                // Padding lanes store too: no real cell reads their entries.
+                const uint32_t lane = p & 63u;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int s = 0; s < 4; ++s) vt[s * TILE + lane] = Vc[k] + psi * delta[s];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const double2 v0 = reinterpret_cast<const double2 *>(vt)[lane];
+                const double2 v1 = reinterpret_cast<const double2 *>(vt)[lane + TILE];
+                double *row = a.FV + fix(p & ~63u, lane >> 5, k) + 2 * (lane & 31u);
+                *reinterpret_cast<double2 *>(row) = v0;
+                *reinterpret_cast<double2 *>(row + 2 * 5 * TILE) = v1;
+            } else if (live) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) a.FV[fix(c, s, k)] = Vc[k] + psi * delta[s];
+            }
+            if (live) {
                 if (a.grad_aux) {
                     a.grad_aux[15 * (size_t)c + k] = g0;
                     a.grad_aux[15 * (size_t)c + k + 5] = g1;
