@@ -44,6 +44,14 @@ namespace nf {
 #ifndef NF_B_GROUP
 #define NF_B_GROUP 1
 #endif
+// sweep B: load face s + 1 while face s is computed (NF_B_GROUP = 1 only),
+// and load the stage's U / U_old at the start instead of after the faces
+#ifndef NF_B_PIPE
+#define NF_B_PIPE 1
+#endif
+#ifndef NF_B_PREU
+#define NF_B_PREU 1
+#endif
 // sweep A reads neighbours in the lane's own 64-cell tile from LDS
 #ifndef NF_A_LDS
 #define NF_A_LDS 1
@@ -736,6 +744,52 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
     CellGeo g;
     NodeView nv;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool zero_old = st.uold_zero;
+    double Up[5], Uop[5];
+    if (NF_B_PREU) {
+        // (stage 1 and a zero U_old read U twice: same lines, no branch)
+        const double *uo = (a.kind == 1 || zero_old) ? a.U : a.Uo;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Up[k] = a.U[tix(c, k)];
+            Uop[k] = uo[tix(c, k)];
+        }
+    }
+#if NF_B_PIPE
+    // face values of both sides, one face at a time, the next face's loads
+    // issued before the current face is computed
+    double Vo4[4][5], Vn4[4][5];
+    auto load_face = [&](int s) {
+        const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
+        const bool bnd = w[s] & 0x80000000u;
+        const double *src = bnd ? a.Gf + tix(idx, 0) : a.FV + fix(idx, sn, 0);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Vo4[s][k] = a.FV[fix(c, s, k)];
+            Vn4[s][k] = src[k * TILE];
+        }
+    };
+    load_face(0);
+    nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
+    cell_geo(m, g);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s + 1 < 4) load_face(s + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t byte = (code >> (8 * s)) & 0xFFu;
+        V3d S;
+        face_geo(m, g, nv, byte, S, nullptr);
+        // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
+        double UO[5], UN[5], F[5];
+        prim_to_cons(Vo4[s], UO);
+        prim_to_cons(Vn4[s], UN);
+        const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
+        rusanov_side(UO, UN, S, mneg, F);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#else
     // face values of both sides, loaded NF_B_GROUP faces at a time; one face
     // at a time measured fastest at 128^3 (1.11 ms vs 1.14 for two faces and
     // 1.20 for all four faces' loads issued up front)
@@ -775,6 +829,7 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+#endif
     if (live && a.check) {
         bool ok = true;
 #pragma unroll
@@ -788,26 +843,32 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         }
     } else {
         const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
-        const bool zero_old = st.uold_zero;
+        if (!NF_B_PREU) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                Up[k] = a.U[tix(c, k)];
+                Uop[k] = a.kind == 1 || zero_old ? 0.0 : a.Uo[tix(c, k)];
+            }
+        }
         double Un[5], Vnew[5];
         // Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:98-132)
         if (a.kind == 1) {
             const double f = rdiv(dt, g.ivol);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Un[k] = a.U[tix(c, k)] + f * R[k];
+            for (int k = 0; k < 5; ++k) Un[k] = Up[k] + f * R[k];
         } else if (a.kind == 2) {
             const double f = rdiv(1.0 / 4.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
-                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
+                const double uo = zero_old ? 0.0 : Uop[k];
+                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * Up[k] + f * R[k];
             }
         } else {
             const double f = rdiv(2.0 / 3.0 * dt, g.ivol);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
-                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
+                const double uo = zero_old ? 0.0 : Uop[k];
+                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * Up[k] + f * R[k];
             }
         }
         cons_to_prim_d(Un, Vnew);
