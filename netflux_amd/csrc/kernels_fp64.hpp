@@ -625,12 +625,12 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
         }
     } else {
         // face normals as the cell adds them: -S where the cell is the
-        // reference's neighbour j (cell j subtracts tmp/vol)
-        V3d Sg[4];
+        // reference's neighbour j (cell j subtracts tmp/vol); in place, as
+        // the time step below uses |S| only
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
-            Sg[s] = {flip(S[s].x, mneg), flip(S[s].y, mneg), flip(S[s].z, mneg)};
+            S[s] = {flip(S[s].x, mneg), flip(S[s].y, mneg), flip(S[s].z, mneg)};
         }
         // One variable at a time: gradient row k, its neighbour min/max, the
         // Barth factor and the face values depend on variable k only.
@@ -648,12 +648,12 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 // 0.5 (V_i + V_j) is symmetric in i, j; the non-owner subtracts
-                // (through Sg, whose sign flip carries through the product
-                // and the division exactly)
+                // (through the flipped S: the sign carries through the
+                // product and the division exactly)
                 const double uf = 0.5 * (Vc[k] + Vn[s][k]);
-                g0 = g0 + rdiv(uf * Sg[s].x, g.ivol);
-                g1 = g1 + rdiv(uf * Sg[s].y, g.ivol);
-                g2 = g2 + rdiv(uf * Sg[s].z, g.ivol);
+                g0 = g0 + rdiv(uf * S[s].x, g.ivol);
+                g1 = g1 + rdiv(uf * S[s].y, g.ivol);
+                g2 = g2 + rdiv(uf * S[s].z, g.ivol);
                 if (ORDER == 2) {
                     // neighbour min/max (include/Numerics.hpp:276-291): the
                     // maximum of {v, v_j} over the faces, one v_j at a time
