@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: FP32 face arithmetic with FP64 state (not the headline; no roofline)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches in the timed region (profiling)")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="pick the sweep launch configuration with the autotuner first")
@@ -113,7 +115,7 @@ def main():
     t0 = time.perf_counter()
     mesh = nf.Mesh.box(args.n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
     U0 = pulse_state(mesh)
-    cfg = pulse_config(mach=0.5, cell_order=args.order)
+    cfg = pulse_config(mach=0.5, cell_order=args.order, precision=args.precision)
     solver = nf.EulerSolver(cfg, mesh)
     N = solver.N
     F = solver.sizes.n_total_faces
@@ -144,15 +146,18 @@ def main():
         kern[name] = {"avg_ms": avg, "calls": calls}
     tot = {k: v["avg_ms"] * v["calls"] for k, v in kern.items()}
     dom = max(tot, key=tot.get)
-    per_launch = {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0}.get(dom, s1)
-    achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.n),
-                "traffic_source": "profiles/sweep_traffic_128cubed.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
-                "kernel": dom,
-                "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
-                "algorithmic_bytes_per_launch": int(per_launch)}
-    step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
+    if args.precision == "fp64":
+        per_launch = {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0}.get(dom, s1)
+        achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.n),
+                    "traffic_source": "profiles/sweep_traffic_128cubed.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                    "kernel": dom,
+                    "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
+                    "algorithmic_bytes_per_launch": int(per_launch)}
+        step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
+    else:
+        roofline, step_frac = None, None  # the byte model above is the FP64 one
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -169,13 +174,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 faces / f64 state",
         "data": "synthetic (generated jittered Kuhn mesh, Gaussian pulse)",
-        "config": {"workload": f"plain FP64 TVD-RK3 step, unit cube {args.n}^3 hex->6 tets "
+        "config": {"workload": f"plain {'FP64' if args.precision == 'fp64' else 'mixed-precision'} TVD-RK3 step, unit cube {args.n}^3 hex->6 tets "
                                f"({N} cells), jitter 0.15, Mach 0.5 + pulse, farfield, CFL 0.5",
                    "cells": N, "faces": F, "graph_steps": args.graph_steps, "cell_order": args.order,
                    "launch": tune},
-        "step_roofline_frac": round(step_frac, 4),
+        "step_roofline_frac": None if step_frac is None else round(step_frac, 4),
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -115,10 +115,20 @@ __device__ __forceinline__ void rusanov_side_f(const float *UO, const float *UN,
     }
 }
 
-__device__ __forceinline__ float barth_phi_f(float delta, float vmax, float vmin, float v) {
-    const float d2 = float(isign(delta)) * (fabsf(delta) + FLT_EPSILON);
-    const float num = (d2 > 0.0f) ? vmax - v : vmin - v;
-    return (d2 != 0.0f) ? fminf(1.0f, num / d2) : 1.0f;
+// Barth limiter of one variable over the four faces with one division per
+// sign group of Delta_2 (the FP64 kernels' barth_psi_d in FP32: the face
+// with the largest |Delta_2| of a group gives the group's smallest quotient,
+// and correctly rounded division is monotone in the divisor)
+__device__ __forceinline__ float barth_psi_f(const float *delta, float vmax, float vmin, float v) {
+    float hi = 0.0f, lo = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        hi = fmaxf(hi, delta[s]);
+        lo = fminf(lo, delta[s]);
+    }
+    const float pp = (hi > 0.0f) ? fminf(1.0f, (vmax - v) / (hi + FLT_EPSILON)) : 1.0f;
+    const float pm = (lo < 0.0f) ? fminf(1.0f, (vmin - v) / (lo - FLT_EPSILON)) : 1.0f;
+    return fminf(pp, pm);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,36 +160,63 @@ template <int ORDER, int B>
 __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
+    __shared__ float ldsv[5 * B];  // [wave][5][64] own states
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
     if (m.st[L.inst].stop) return;
     const uint32_t c = L.c;
     const bool live = L.live;
+    const uint32_t p = L.lb * B + threadIdx.x;  // device position (padding included)
 
     const uint32_t code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
+    const NodeTableLoad ntl = node_table_issue(m, p);
     __builtin_amdgcn_sched_barrier(0);
     float Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    // neighbours in the wave's own tile come from an LDS copy of the tile;
+    // the gathers are exec-masked to the other lanes (as in the FP64 sweep)
     float Vn[4][5];
+    bool loc[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        loc[s] = !(w[s] & 0x80000000u) && ((w[s] & 0x1FFFFFFFu) >> 6) == (p >> 6);
     if (ORDER != 0) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const uint32_t idx = w[s] & 0x1FFFFFFFu;
-            const float *src = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
+        for (int s = 0; s < 4; ++s)
+            if (!loc[s]) {
+                const uint32_t idx = w[s] & 0x1FFFFFFFu;
+                const float *src = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * TILE];
-        }
+                for (int k = 0; k < 5; ++k)
+                    Vn[s][k] = *(const __attribute__((address_space(1))) float *)(src + k * TILE);
+            }
+    }
+    float *vt = ldsv + (threadIdx.x >> 6) * 5 * TILE;
+    if (ORDER != 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vt[k * TILE + (p & 63u)] = Vc[k];
     }
     CellGeo g;
-    const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
+    // (its wave-level fences also order the state writes above before the
+    // reads below)
+    const NodeView nv = node_table_finish<B>(m, p, ntl, lds, g);
     cell_geo(m, g);
     V3f S[4], r[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) face_geo_mx(nv, g, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     const float ivol = (float)(1.0 / g.vol);
+    __builtin_amdgcn_sched_barrier(0);
+    if (ORDER != 0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (loc[s]) {
+                const uint32_t j = w[s] & 63u;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) Vn[s][k] = vt[k * TILE + j];
+            }
+    }
 
     if (ORDER == 0) {
         if (live) {
@@ -191,7 +228,7 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     } else {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            float g0 = 0.f, g1 = 0.f, g2 = 0.f, vmax = -FLT_MAX, vmin = FLT_MAX;
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f, vmax = Vc[k], vmin = Vc[k];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const uint32_t mneg = ((code >> (8 * s + 7)) & 1u) ? 0u : 0x80000000u;
@@ -200,17 +237,14 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
                 g1 += uf * S[s].y;
                 g2 += uf * S[s].z;
                 if (ORDER == 2) {
-                    vmax = fmaxf(vmax, fmaxf(Vc[k], Vn[s][k]));
-                    vmin = fminf(vmin, fminf(Vc[k], Vn[s][k]));
+                    vmax = fmaxf(vmax, Vn[s][k]);
+                    vmin = fminf(vmin, Vn[s][k]);
                 }
             }
             float delta[4];
-            float psi = (ORDER == 2) ? FLT_MAX : 1.0f;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                delta[s] = g0 * r[s].x + g1 * r[s].y + g2 * r[s].z;
-                if (ORDER == 2) psi = fminf(psi, barth_phi_f(delta[s], vmax, vmin, Vc[k]));
-            }
+            for (int s = 0; s < 4; ++s) delta[s] = g0 * r[s].x + g1 * r[s].y + g2 * r[s].z;
+            const float psi = (ORDER == 2) ? barth_psi_f(delta, vmax, vmin, Vc[k]) : 1.0f;
             if (live) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) a.FV[fix(c, s, k)] = Vc[k] + psi * delta[s];
@@ -250,6 +284,18 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
     const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
     const NodeTableLoad ntl = node_table_issue(m, L.lb * B + threadIdx.x);
     __builtin_amdgcn_sched_barrier(0);
+    // the stage state first, outside the stage-kind branches (stage 1 and a
+    // zero U_old read U twice: same lines, no branch)
+    const bool zero_old = st.uold_zero;
+    double Up[5], Uop[5];
+    {
+        const double *uo = (a.kind == 1 || zero_old) ? a.U : a.Uo;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Up[k] = a.U[tix(c, k)];
+            Uop[k] = uo[tix(c, k)];
+        }
+    }
     float Vo4[4][5], Vn4[4][5];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -292,25 +338,24 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
         }
     } else {
         const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
-        const bool zero_old = st.uold_zero;
         double Un[5], Vnew[5];
         if (a.kind == 1) {
             const double f = dt / g.vol;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Un[k] = a.U[tix(c, k)] + f * R[k];
+            for (int k = 0; k < 5; ++k) Un[k] = Up[k] + f * R[k];
         } else if (a.kind == 2) {
             const double f = 1.0 / 4.0 * dt / g.vol;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
-                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * a.U[tix(c, k)] + f * R[k];
+                const double uo = zero_old ? 0.0 : Uop[k];
+                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * Up[k] + f * R[k];
             }
         } else {
             const double f = 2.0 / 3.0 * dt / g.vol;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : a.Uo[tix(c, k)];
-                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * a.U[tix(c, k)] + f * R[k];
+                const double uo = zero_old ? 0.0 : Uop[k];
+                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * Up[k] + f * R[k];
             }
         }
         cons_to_prim_d(Un, Vnew);

@@ -184,12 +184,4 @@ NF_HD void ghost_value(int bc, const T *Vd, Vec3T<T> S, const FarField<T> &fs, T
     }
 }
 
-// Barth face contribution for one variable (include/Numerics.hpp:190-206)
-template <typename T> NF_HD T barth_phi(T delta, T vmax, T vmin, T v) {
-    const T d2 = T(isign(delta)) * (sabs(delta) + Consts<T>::EPS);
-    if (d2 > T(0)) return smin(T(1), (vmax - v) / d2);
-    if (d2 < T(0)) return smin(T(1), (vmin - v) / d2);
-    return T(1);
-}
-
 }  // namespace nf
