@@ -498,9 +498,10 @@ struct GhostArgs {
     const uint8_t *bbc;          // boundary type
     const uint16_t *binst;       // instance
     uint32_t nb;                 // boundary faces
-    const double *src;           // V [tile][5][64] (cell) or FV [tile][4][5][64] (face)
+    const double *src;           // V or U [tile][5][64] (cell) or FV [tile][4][5][64] (face)
     double *out;                 // [nb/64][5][64], tiled like V
     const FarField<double> *fs;  // [instances]
+    int src_is_u;                // cell ghosts: src is U, the primvars are prim(U)
 };
 
 // FACE = false: ghost of the cell-centre state (set_constant_ghost_values,
@@ -522,6 +523,10 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
     double Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];
+    if (!FACE && a.src_is_u) {
+        const double Ud[5] = {Vd[0], Vd[1], Vd[2], Vd[3], Vd[4]};
+        cons_to_prim_d(Ud, Vd);
+    }
     ghost_d((int)a.bbc[b], Vd, S, a.fs[inst], Vg);
     if (live && !m.st[inst].stop) {
 #pragma unroll
@@ -531,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
 
 // ---------------------------------------------------------------------------
 struct SweepAArgs {
-    const double *V;   // [tile][5][64]
+    const double *V;   // [tile][5][64] primvars (FROM_U = false) or the state U (FROM_U)
     const double *Gc;  // [nb/64][5][64] ghost of the cell-centre state
     double *FV;        // [tile][4][5][64]
     double *dt_part;   // per-block CFL dt minimum, or null
@@ -542,8 +547,12 @@ struct SweepAArgs {
     double *psi_aux;   // [N][5] or null
 };
 
-// ORDER: 0 first order, 1 second order without limiter, 2 second order Barth
-template <int ORDER, int B>
+// ORDER: 0 first order, 1 second order without limiter, 2 second order Barth.
+// FROM_U: a.V holds the stage's state U and the sweep evaluates the
+// primvars itself (cons_to_prim of the own cell and of the neighbours it
+// gathers; neighbours in the tile come from the LDS copy of the own values),
+// so that sweep B need not store them.
+template <int ORDER, int B, bool FROM_U>
 __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
@@ -562,7 +571,7 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
     __builtin_amdgcn_sched_barrier(0);
     double Vc[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
+    for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];  // (U if FROM_U: converted below)
     // neighbour (or ghost) states; ghosts are tiled like V, so every source
     // has the same stride and the five loads share one address.  A neighbour
     // in the lane's own tile (most of them in Morton order) is read from the
@@ -592,6 +601,10 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
                 for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k) Vn[s][k] = ldg(nsrc[s] + k * TILE);
             }
     }
+    if (FROM_U) {
+        const double Uc[5] = {Vc[0], Vc[1], Vc[2], Vc[3], Vc[4]};
+        cons_to_prim_d(Uc, Vc);
+    }
     double *vt = ldsv + (NF_A_LDS ? (threadIdx.x >> 6) * 5 * TILE : 0);
     if (NF_A_LDS && ORDER != 0) {
 #pragma unroll
@@ -606,6 +619,15 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
 #pragma unroll
     for (int s = 0; s < 4; ++s) face_geo(m, g, nv, (code >> (8 * s)) & 0xFFu, S[s], &r[s]);
     __builtin_amdgcn_sched_barrier(0);
+    if (FROM_U && ORDER != 0) {
+        // gathered neighbour states to primvars (ghost values already are)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (!loc[s] && !(w[s] & 0x80000000u)) {
+                const double Un[5] = {Vn[s][0], Vn[s][1], Vn[s][2], Vn[s][3], Vn[s][4]};
+                cons_to_prim_d(Un, Vn[s]);
+            }
+    }
     if (NF_A_LDS && ORDER != 0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -717,7 +739,7 @@ struct SweepBArgs {
     const double *U;        // [tile][5][64] state entering the stage
     const double *Uo;       // U_old (U^n)
     double *Uout;           // (may alias U)
-    double *Vout;           // prim(Uout)
+    double *Vout;           // prim(Uout), or null (the next sweep A evaluates it)
     const double *dt_cell;  // per-cell step (local time stepping), or null
     int kind;               // 1: U + dt/vol R, 2: RK3 stage 2, 3: RK3 stage 3
     double *res_part;       // per-block sum of R_rho^2, or null
@@ -874,9 +896,10 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         cons_to_prim_d(Un, Vnew);
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                a.Uout[tix(c, k)] = Un[k];
-                a.Vout[tix(c, k)] = Vnew[k];
+            for (int k = 0; k < 5; ++k) a.Uout[tix(c, k)] = Un[k];
+            if (a.Vout) {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a.Vout[tix(c, k)] = Vnew[k];
             }
             if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
         }

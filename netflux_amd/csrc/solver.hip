@@ -191,6 +191,9 @@ struct Inst {
     uint32_t wall0 = 0, nwall = 0;  // wall-face centroids of the instance
     double rref = 0.0;              // residual reference of the drop (0: r_0)
     bool warm = false;              // state from the network, not yet fallen back
+    // FP64 primvars V of the instance differ from prim(U): steps do not store
+    // them (each sweep A evaluates prim(U) itself); refreshed where V is read
+    bool v_stale = false;
 };
 
 // reduction chunks of whole instances for one sweep block size
@@ -319,7 +322,21 @@ int flush_timing(nf_ctx *ctx) {
     return NF_OK;
 }
 
-int run_ghost(nf_ctx *ctx, bool face) {
+// V = prim(U) for the instances whose FP64 primvars are stale (what a step
+// that stored them would have left)
+int refresh_v(nf_ctx *ctx) {
+    for (Inst &I : ctx->inst) {
+        if (!I.v_stale) continue;
+        hipLaunchKernelGGL(k_prim, dim3(I.Npad / BLOCK), dim3(BLOCK), 0, ctx->stream,
+                           (const double *)(ctx->Ua + (size_t)5 * I.cell0), ctx->V + (size_t)5 * I.cell0, I.Npad);
+        HIPC(hipGetLastError());
+        I.v_stale = false;
+    }
+    return NF_OK;
+}
+
+// U: cell ghosts from prim(U) of this state instead of the stored primvars
+int run_ghost(nf_ctx *ctx, bool face, const double *U = nullptr) {
     if (ctx->nbf == 0) return NF_OK;
     const dim3 g((ctx->nbf + BLOCK - 1) / BLOCK), b(BLOCK);
     const DevMesh m = devmesh(ctx);
@@ -329,8 +346,8 @@ int run_ghost(nf_ctx *ctx, bool face) {
         return launch_fn(ctx, face ? "ghost_face_mx" : "ghost_cell_mx",
                          [&] { return mx::launch_ghost_mx(face, g, ctx->stream, m, a); });
     }
-    const GhostArgs a{ctx->bface, ctx->bbc, ctx->binst, ctx->nbf, face ? ctx->FV : ctx->V,
-                      face ? ctx->Gf : ctx->Gc, ctx->fs};
+    const GhostArgs a{ctx->bface, ctx->bbc, ctx->binst, ctx->nbf, face ? ctx->FV : (U ? U : ctx->V),
+                      face ? ctx->Gf : ctx->Gc, ctx->fs, (!face && U) ? 1 : 0};
     if (face) return LAUNCH("ghost_face", k_ghost<true>, g, b, m, a);
     return LAUNCH("ghost_cell", k_ghost<false>, g, b, m, a);
 }
@@ -340,21 +357,23 @@ int order_of(const nf_ctx *ctx) {
     return ctx->prm.limiter == NF_LIMITER_NONE ? 1 : 2;
 }
 
-template <int B>
+template <int B, bool FROM_U>
 hipError_t launch_sweepA(hipStream_t s, uint32_t ncell, int order, const DevMesh &m, const SweepAArgs &a) {
     const dim3 g(ncell / B), b(B);
-    if (order == 0) hipLaunchKernelGGL((k_sweepA<0, B>), g, b, 0, s, m, a);
-    else if (order == 1) hipLaunchKernelGGL((k_sweepA<1, B>), g, b, 0, s, m, a);
-    else hipLaunchKernelGGL((k_sweepA<2, B>), g, b, 0, s, m, a);
+    if (order == 0) hipLaunchKernelGGL((k_sweepA<0, B, FROM_U>), g, b, 0, s, m, a);
+    else if (order == 1) hipLaunchKernelGGL((k_sweepA<1, B, FROM_U>), g, b, 0, s, m, a);
+    else hipLaunchKernelGGL((k_sweepA<2, B, FROM_U>), g, b, 0, s, m, a);
     return hipGetLastError();
 }
 
-// sweep A of the current primvars.  dt: per-block CFL minima into dt_part;
-// dtc: per-cell CFL steps into dt_cell (local time stepping); aux: keep the
-// gradient and limiter fields.
-int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dtc = false) {
+// sweep A of the current primvars, or (U set, FP64) of prim(U) of that state.
+// dt: per-block CFL minima into dt_part; dtc: per-cell CFL steps into dt_cell
+// (local time stepping); aux: keep the gradient and limiter fields.
+int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dtc = false, const double *U = nullptr) {
     int rc;
-    if ((rc = run_ghost(ctx, false))) return rc;
+    if (ctx->mixed) U = nullptr;
+    if (!U && !ctx->mixed && (rc = refresh_v(ctx))) return rc;
+    if ((rc = run_ghost(ctx, false, U))) return rc;
     const DevMesh m = devmesh(ctx);
     const int order = order_of(ctx);
     if (ctx->mixed) {
@@ -365,12 +384,15 @@ int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dtc = false) {
             return mx::launch_sweepA_mx(order, ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
         });
     }
-    const SweepAArgs a{ctx->V, ctx->Gc, ctx->FV, dt ? ctx->dt_part : nullptr, dtc ? ctx->dt_cell : nullptr,
+    const SweepAArgs a{U ? U : ctx->V, ctx->Gc, ctx->FV, dt ? ctx->dt_part : nullptr, dtc ? ctx->dt_cell : nullptr,
                        ctx->prm.CFL, ctx->xcd_remap, aux ? ctx->grad_aux : nullptr, aux ? ctx->psi_aux : nullptr};
     const char *name = order == 2 ? "sweepA_barth" : (order == 1 ? "sweepA_nolim" : "sweepA_first");
     return launch_fn(ctx, name, [&] {
-        return ctx->block == 128 ? launch_sweepA<128>(ctx->stream, ctx->Ncell, order, m, a)
-                                 : launch_sweepA<256>(ctx->stream, ctx->Ncell, order, m, a);
+        if (U)
+            return ctx->block == 128 ? launch_sweepA<128, true>(ctx->stream, ctx->Ncell, order, m, a)
+                                     : launch_sweepA<256, true>(ctx->stream, ctx->Ncell, order, m, a);
+        return ctx->block == 128 ? launch_sweepA<128, false>(ctx->stream, ctx->Ncell, order, m, a)
+                                 : launch_sweepA<256, false>(ctx->stream, ctx->Ncell, order, m, a);
     });
 }
 
@@ -389,7 +411,8 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, double *Uout, bool res, b
             return mx::launch_sweepB_mx(ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
         });
     }
-    const SweepBArgs a{ctx->FV, ctx->Gf, U, ctx->Ua, Uout, ctx->V, dtc, kind, res ? ctx->res_part : nullptr,
+    // (FP64: prim(Uout) is not stored; the next sweep A evaluates it from U)
+    const SweepBArgs a{ctx->FV, ctx->Gf, U, ctx->Ua, Uout, nullptr, dtc, kind, res ? ctx->res_part : nullptr,
                        ctx->prm.check_physical_validity, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
     return launch_fn(ctx, "sweepB", [&] {
         if (ctx->block == 128)
@@ -430,15 +453,18 @@ int enqueue_step(nf_ctx *ctx, double drop) {
     int rc;
     const bool rk3 = ctx->prm.time_scheme == NF_TVD_RK3;
     const bool lts = ctx->prm.local_time_step;
-    if ((rc = run_sweepA(ctx, !rk3 || lts, false, lts))) return rc;
+    // each stage's sweep A evaluates the primvars of its input state
+    if ((rc = run_sweepA(ctx, !rk3 || lts, false, lts, ctx->Ua))) return rc;
     if ((rc = run_sweepB(ctx, 1, ctx->Ua, rk3 ? ctx->Ub : ctx->Ua, true, false))) return rc;
     if ((rc = reduce_res(ctx, drop))) return rc;
     if (rk3) {
-        if ((rc = run_sweepA(ctx, false, false))) return rc;
+        if ((rc = run_sweepA(ctx, false, false, false, ctx->Ub))) return rc;
         if ((rc = run_sweepB(ctx, 2, ctx->Ub, ctx->Ub, false, false))) return rc;
-        if ((rc = run_sweepA(ctx, !lts, false))) return rc;
+        if ((rc = run_sweepA(ctx, !lts, false, false, ctx->Ub))) return rc;
         if ((rc = run_sweepB(ctx, 3, ctx->Ub, ctx->Ua, false, false))) return rc;
     }
+    if (!ctx->mixed)
+        for (Inst &I : ctx->inst) I.v_stale = true;
     return reduce_dt(ctx, 1);
 }
 
@@ -563,6 +589,7 @@ int after_state_set(nf_ctx *ctx, int i, bool uold_zero) {
     HIPC(hipStreamSynchronize(ctx->stream));
     I.dt_valid = false;
     I.warm = false;
+    I.v_stale = false;
     ctx->aux_valid = false;
     return NF_OK;
 }
@@ -576,6 +603,7 @@ int prim_all(nf_ctx *ctx) {
                            ctx->V, ctx->Ncell);
         HIPC(hipGetLastError());
     }
+    for (Inst &I : ctx->inst) I.v_stale = false;
     return NF_OK;
 }
 
@@ -1051,7 +1079,7 @@ struct Snapshot {
     std::vector<double> U, V;
     std::vector<float> V32;
     std::vector<InstState> st;
-    std::vector<bool> dt_valid;
+    std::vector<bool> dt_valid, v_stale;
 };
 
 int save(nf_ctx *ctx, Snapshot &s) {
@@ -1068,7 +1096,11 @@ int save(nf_ctx *ctx, Snapshot &s) {
         HIPC(hipMemcpy(s.V.data(), ctx->V, n5 * sizeof(double), hipMemcpyDeviceToHost));
     }
     s.dt_valid.clear();
-    for (auto &I : ctx->inst) s.dt_valid.push_back(I.dt_valid);
+    s.v_stale.clear();
+    for (auto &I : ctx->inst) {
+        s.dt_valid.push_back(I.dt_valid);
+        s.v_stale.push_back(I.v_stale);
+    }
     return NF_OK;
 }
 
@@ -1079,7 +1111,10 @@ int restore(nf_ctx *ctx, const Snapshot &s) {
     if (ctx->mixed) HIPC(hipMemcpy(ctx->V32, s.V32.data(), n5 * sizeof(float), hipMemcpyHostToDevice));
     else HIPC(hipMemcpy(ctx->V, s.V.data(), n5 * sizeof(double), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(ctx->st, s.st.data(), s.st.size() * sizeof(InstState), hipMemcpyHostToDevice));
-    for (size_t i = 0; i < ctx->inst.size(); ++i) ctx->inst[i].dt_valid = s.dt_valid[i];
+    for (size_t i = 0; i < ctx->inst.size(); ++i) {
+        ctx->inst[i].dt_valid = s.dt_valid[i];
+        ctx->inst[i].v_stale = s.v_stale[i];
+    }
     ctx->aux_valid = false;
     return NF_OK;
 }
@@ -1237,9 +1272,12 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
 int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
     if (!single(ctx) || !out) return NF_E_ARG;
     switch (which) {
-    case NF_FIELD_PRIM:
+    case NF_FIELD_PRIM: {
         if (ctx->mixed) return download_state(ctx, 0, (const float *)ctx->V32, out);
+        int rc;
+        if ((rc = refresh_v(ctx))) return rc;
         return download_state(ctx, 0, (const double *)ctx->V, out);
+    }
     case NF_FIELD_STATE_OLD: {
         std::vector<InstState> h;
         int rc;

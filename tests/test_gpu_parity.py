@@ -176,3 +176,63 @@ def test_flux_balance_between_steps_leaves_run_unchanged(scheme):
         gpu.step(cfg)
         ora.step(dt)
     np.testing.assert_array_equal(gpu.get_solver_data().get_solution(), ora.get("U"))
+
+
+def _quantised_state(mesh, seed):
+    """Primitive values drawn from two or three levels per variable, so that
+    many neighbour differences, limiter numerators and gradients are exact
+    zeros or exact ties."""
+    from tests.cases import prim_to_cons
+    rng = np.random.default_rng(seed)
+    N = mesh.n_cells
+    V = np.zeros((N, 5))
+    V[:, 0] = rng.choice([1.0, 1.5], N)
+    V[:, 1] = rng.choice([0.0, 0.25, -0.25], N)
+    V[:, 2] = rng.choice([0.0, 0.5], N)
+    V[:, 3] = 0.0
+    V[:, 4] = rng.choice([1.0, 2.0], N)
+    return prim_to_cons(V)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_limiter_ties_and_zeros_match_oracle(seed):
+    """Barth with exact zero numerators, ties between faces and zero
+    gradient rows (the grouped limiter's edge cases) against the oracle's
+    face-by-face limiter, and 10 RK3 steps from there."""
+    mesh = nf.Mesh.box(8, jitter=0.1, seed=seed)
+    cfg = sod_config()
+    gpu, ora = _pair(mesh, cfg, _quantised_state(mesh, seed))
+    gpu.eval_flux_balance()
+    ora.eval_flux_balance()
+    d = gpu.get_solver_data()
+    np.testing.assert_array_equal(d.get_primvars_limiter(), ora.get("psi"))
+    np.testing.assert_array_equal(d.get_primvars_gradient(), ora.get("grad"))
+    np.testing.assert_array_equal(d.get_flux_balance(), ora.get("R"))
+    for _ in range(10):
+        gpu.calc_timestep(cfg)
+        gpu.step(cfg)
+    ora.run(10)
+    np.testing.assert_array_equal(d.get_solution(), ora.get("U"))
+
+
+def test_limiter_tiny_gradients_match_oracle():
+    """Velocity components of order 1e-250: limiter denominators Delta_2 =
+    d + eps round to eps, which the grouped limiter must see as positive."""
+    from tests.cases import prim_to_cons
+    mesh = nf.Mesh.box(6, jitter=0.1, seed=3)
+    N = mesh.n_cells
+    rng = np.random.default_rng(5)
+    V = np.zeros((N, 5))
+    V[:, 0] = 1.0
+    V[:, 1] = 1e-250 * rng.random(N)
+    V[:, 2] = -1e-250 * rng.random(N)
+    V[:, 4] = 1.0
+    cfg = sod_config()
+    gpu, ora = _pair(mesh, cfg, prim_to_cons(V))
+    gpu.eval_flux_balance()
+    ora.eval_flux_balance()
+    d = gpu.get_solver_data()
+    psi = ora.get("psi")
+    assert np.any((psi[:, 1] > 0) & (psi[:, 1] < 1))  # the tiny rows are limited
+    np.testing.assert_array_equal(d.get_primvars_limiter(), psi)
+    np.testing.assert_array_equal(d.get_primvars_gradient(), ora.get("grad"))
