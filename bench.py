@@ -65,13 +65,29 @@ def cpu_baseline(sample_n, steps):
     cfg = pulse_config()
     ora = Oracle.from_mesh(mesh, cfg)
     ora.set_state(pulse_state(mesh))
-    t0 = time.perf_counter()
-    ora.run(steps)
-    t = time.perf_counter() - t0
+    # one core, as the reference runs (serial); the oracle is built with the
+    # reference's release flags (g++ -O3 -flto -DNDEBUG, no -march)
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[0]})
+    try:
+        t0 = time.perf_counter()
+        ora.run(steps)
+        t = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, set(cpus))
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     N = mesh.n_cells
     return {"value": N * 3 * steps / t, "unit": "cell-stage-updates/s", "cores": 1, "kind": "port",
             "sample": f"oracle, {sample_n}^3x6 = {N} cells, {steps} TVD-RK3 steps, same workload family "
-                      f"(jitter 0.15, pulse, farfield), one thread, {t:.1f} s"}
+                      f"(jitter 0.15, pulse, farfield), one thread pinned to core {cpus[0]} of "
+                      f"{len(cpus)} available, {model}, {t:.1f} s"}
 
 
 def main():
