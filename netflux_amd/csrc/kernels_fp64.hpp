@@ -547,60 +547,74 @@ struct SweepAArgs {
     double *psi_aux;   // [N][5] or null
 };
 
+// One 64-cell tile of sweep A: its loads (tileA_issue) and the rest
+// (tileA_run).
+//
 // ORDER: 0 first order, 1 second order without limiter, 2 second order Barth.
 // FROM_U: a.V holds the stage's state U and the sweep evaluates the
 // primvars itself (cons_to_prim of the own cell and of the neighbours it
 // gathers; neighbours in the tile come from the LDS copy of the own values),
 // so that sweep B need not store them.
-template <int ORDER, int B, bool FROM_U>
-__global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
-    __shared__ double sh[B / 64];
-    __shared__ double lds[12 * B];
-    __shared__ __align__(16) double ldsv[NF_A_LDS ? 5 * B : 2];  // [wave][5][64] own states
-    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
-    if (m.st[L.inst].stop) return;
-    const uint32_t c = L.c;
-    const bool live = L.live;
-    const uint32_t p = L.lb * B + threadIdx.x;  // device position (padding included)
+struct TileA {
+    uint32_t c, p;  // cell (clamped for padding lanes), device position
+    bool live;
+    uint32_t code, w[4];
+    NodeTableLoad ntl;
+    double Vc[5], Vn[4][5];
+    const double *nsrc[4];
+    bool loc[4];
+};
 
+template <int ORDER, bool FROM_U>
+__device__ __forceinline__ void tileA_issue(const DevMesh &m, const SweepAArgs &a, TileA &t) {
+    const uint32_t c = t.c, p = t.p;
     // all loads first: own state, neighbour states (or ghost states), nodes
-    const uint32_t code = m.fcode[c];
+    t.code = m.fcode[c];
     const uint4 w4 = *reinterpret_cast<const uint4 *>(m.nbr + 4 * (size_t)c);
-    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
-    const NodeTableLoad ntl = node_table_issue(m, p);
+    t.w[0] = w4.x, t.w[1] = w4.y, t.w[2] = w4.z, t.w[3] = w4.w;
+    t.ntl = node_table_issue(m, p);
     __builtin_amdgcn_sched_barrier(0);
-    double Vc[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];  // (U if FROM_U: converted below)
+    for (int k = 0; k < 5; ++k) t.Vc[k] = a.V[tix(c, k)];  // (U if FROM_U: converted in tileA_run)
     // neighbour (or ghost) states; ghosts are tiled like V, so every source
     // has the same stride and the five loads share one address.  A neighbour
     // in the lane's own tile (most of them in Morton order) is read from the
     // wave's copy of the tile in LDS instead (NF_A_LDS): the gathers are
-    // exec-masked to the other lanes, which is what saves time (the
-    // texture path's work grows with the active lanes).  Every cell of the
-    // tile is a lane of this wave and live, since neighbours are real cells.
-    // The gathers go first: the waits for the node and state loads then
-    // cover them too (masked loads cannot be counted), which measured
+    // exec-masked to the other lanes, which is what saves time (the texture
+    // path's work grows with the lanes that access memory).  Every cell of
+    // the tile is a lane of this wave and live, since neighbours are real
+    // cells.  The gathers go first: the waits for the node and state loads
+    // then cover them too (masked loads cannot be counted), which measured
     // faster than issuing them after the LDS copies (1.04 vs 0.93 ms).
     // NF_A_PREFETCH = 1 loads variable k + 1 while variable k is processed
     // instead of all five up front (fewer registers, less parallelism)
-    double Vn[4][5];
-    const double *nsrc[4];
-    bool loc[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const uint32_t idx = w[s] & 0x1FFFFFFFu;
-        nsrc[s] = ((w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
-        loc[s] = NF_A_LDS && !(w[s] & 0x80000000u) && (idx >> 6) == (p >> 6);
+        const uint32_t idx = t.w[s] & 0x1FFFFFFFu;
+        t.nsrc[s] = ((t.w[s] & 0x80000000u) ? a.Gc : a.V) + tix(idx, 0);
+        t.loc[s] = NF_A_LDS && !(t.w[s] & 0x80000000u) && (idx >> 6) == (p >> 6);
     }
     if (ORDER != 0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-            if (!loc[s]) {
+            if (!t.loc[s]) {
 #pragma unroll
-                for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k) Vn[s][k] = ldg(nsrc[s] + k * TILE);
+                for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k) t.Vn[s][k] = ldg(t.nsrc[s] + k * TILE);
             }
     }
+}
+
+// the tile's gradient, limiter and face values; returns the cell's CFL step
+// (DBL_MAX for padding lanes or without a.dt_part)
+template <int ORDER, int B, bool FROM_U>
+__device__ __forceinline__ double tileA_run(const DevMesh &m, const SweepAArgs &a, TileA &t, double *lds,
+                                            double *ldsv) {
+    const uint32_t c = t.c, p = t.p, code = t.code;
+    const bool live = t.live;
+    const uint32_t *w = t.w;
+    const bool *loc = t.loc;
+    double *Vc = t.Vc;
+    double(*Vn)[5] = t.Vn;
     if (FROM_U) {
         const double Uc[5] = {Vc[0], Vc[1], Vc[2], Vc[3], Vc[4]};
         cons_to_prim_d(Uc, Vc);
@@ -613,7 +627,7 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
     CellGeo g;
     // (its wave-level fences also order the state writes above before the
     // reads below)
-    const NodeView nv = node_table_finish<B>(m, p, ntl, lds, g);
+    const NodeView nv = node_table_finish<B>(m, p, t.ntl, lds, g);
     cell_geo(m, g);
     V3d S[4], r[4];
 #pragma unroll
@@ -661,7 +675,7 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
             if (NF_A_PREFETCH && k + 1 < 5) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
-                    if (!loc[s]) Vn[s][k + 1] = nsrc[s][(k + 1) * TILE];
+                    if (!loc[s]) Vn[s][k + 1] = t.nsrc[s][(k + 1) * TILE];
             }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
@@ -723,10 +737,30 @@ __global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArg
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+    double dtc = DBL_MAX;
     if (a.dt_part) {
-        const double t = cell_dt(Vc, S, w, g.vol, a.CFL);
-        const double dtc = (live && t < DBL_MAX) ? t : DBL_MAX;  // NaN/+inf never win std::min
-        if (a.dt_cell && live) a.dt_cell[c] = t;
+        const double tc = cell_dt(Vc, S, w, g.vol, a.CFL);
+        dtc = (live && tc < DBL_MAX) ? tc : DBL_MAX;  // NaN/+inf never win std::min
+        if (a.dt_cell && live) a.dt_cell[c] = tc;
+    }
+    return dtc;
+}
+
+template <int ORDER, int B, bool FROM_U>
+__global__ __launch_bounds__(B) NF_SWEEP_ATTR void k_sweepA(DevMesh m, SweepAArgs a) {
+    __shared__ double sh[B / 64];
+    __shared__ double lds[12 * B];
+    __shared__ __align__(16) double ldsv[NF_A_LDS ? 5 * B : 2];  // [wave][5][64] own states
+    const LaneCell L = lane_cell<B>(m, a.xcd_remap);
+    if (m.st[L.inst].stop) return;
+    TileA t;
+    t.c = L.c;
+    t.live = L.live;
+    t.p = L.lb * B + threadIdx.x;
+    tileA_issue<ORDER, FROM_U>(m, a, t);
+    __builtin_amdgcn_sched_barrier(0);
+    const double dtc = tileA_run<ORDER, B, FROM_U>(m, a, t, lds, ldsv);
+    if (a.dt_part) {
         const double v = block_reduce<true, B>(dtc, sh);
         if (threadIdx.x == 0) a.dt_part[L.lb] = v;
     }
