@@ -1003,6 +1003,8 @@ int run_loop(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
                 ctx->graph_xcd = ctx->xcd_remap;
             }
             HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
+            if (!ctx->mixed)  // (the capture marked them once; a replay steps again)
+                for (Inst &I : ctx->inst) I.v_stale = true;
         } else {
             for (int s = 0; s < chunk; ++s)
                 if ((rc = enqueue_step(ctx, drop))) return rc;

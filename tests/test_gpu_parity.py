@@ -236,3 +236,19 @@ def test_limiter_tiny_gradients_match_oracle():
     assert np.any((psi[:, 1] > 0) & (psi[:, 1] < 1))  # the tiny rows are limited
     np.testing.assert_array_equal(d.get_primvars_limiter(), psi)
     np.testing.assert_array_equal(d.get_primvars_gradient(), ora.get("grad"))
+
+
+def test_primvars_export_after_graph_replays():
+    """Steps do not store the primvars; the export evaluates prim(U) of the
+    current state, also after graph replays that follow an earlier export."""
+    mesh = nf.Mesh.box(6, jitter=0.1, seed=8)
+    cfg = sod_config()
+    a = nf.EulerSolver(cfg, mesh)
+    a.set_state(sod_state(mesh))
+    a.run(8, check_every=4, use_graph=True)  # capture + replay
+    a.get_solver_data().get_primvars()
+    a.run(8, check_every=4, use_graph=True)  # replays only
+    Va = a.get_solver_data().get_primvars()
+    b = nf.EulerSolver(cfg, mesh)
+    b.set_state(a.get_solver_data().get_solution())
+    np.testing.assert_array_equal(Va, b.get_solver_data().get_primvars())
