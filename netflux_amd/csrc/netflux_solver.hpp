@@ -10,6 +10,9 @@
 //                          get_primvars_gradient(), get_primvars_limiter(),
 //                          get_flux_balance() as AoS VecField copies
 //   netflux::Driver        the time loop of src/Driver.cpp:31-57 (no VTK)
+//   netflux::Mesh          owner of a library mesh: SU2 files
+//                          (PrimalGrid::read_su2_mesh, src/geometry/
+//                          PrimalGrid.cpp:140-244) or generated boxes
 //
 // Errors are thrown as std::runtime_error carrying the reference's messages
 // (src/Solver.cpp:267-268, src/SolverData.cpp:73-78).
@@ -201,6 +204,44 @@ class BatchSolver {
         return r;
     }
     nf_ctx *handle() const { return ctx_; }
+};
+
+// A library-owned primal mesh.  read_su2 maps each marker tag to a boundary
+// type by name; an unnamed marker or a non-tetrahedral element throws with
+// the reader's message.
+class Mesh {
+    nf_mesh *m_ = nullptr;
+
+  public:
+    explicit Mesh(nf_mesh *m) : m_(m) {}
+    Mesh(const Mesh &) = delete;
+    Mesh &operator=(const Mesh &) = delete;
+    Mesh(Mesh &&o) noexcept : m_(o.m_) { o.m_ = nullptr; }
+    ~Mesh() {
+        if (m_) nf_mesh_destroy(m_);
+    }
+    static Mesh read_su2(const std::string &path, const std::vector<std::string> &names,
+                         const std::vector<BoundaryType> &types) {
+        if (names.size() != types.size()) throw std::runtime_error("read_su2: one boundary type per patch name");
+        std::vector<const char *> n;
+        std::vector<int32_t> bc;
+        for (size_t i = 0; i < names.size(); ++i) {
+            n.push_back(names[i].c_str());
+            bc.push_back(types[i] == BoundaryType::NoSlipWall ? NF_BC_NOSLIP_WALL
+                         : types[i] == BoundaryType::SlipWall ? NF_BC_SLIP_WALL
+                                                              : NF_BC_FARFIELD);
+        }
+        nf_mesh *m = nullptr;
+        if (nf_mesh_read_su2(path.c_str(), (int32_t)n.size(), n.data(), bc.data(), &m) != NF_OK)
+            throw std::runtime_error("Error reading su2 mesh '" + path + "', " + nf_mesh_last_error());
+        return Mesh(m);
+    }
+    nf_mesh_desc desc() const {
+        nf_mesh_desc d;
+        if (nf_mesh_get_desc(m_, &d) != NF_OK) throw std::runtime_error("nf_mesh_get_desc failed");
+        return d;
+    }
+    nf_mesh *handle() const { return m_; }
 };
 
 // src/Driver.cpp:31-57 without output
