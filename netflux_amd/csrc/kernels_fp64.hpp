@@ -38,20 +38,6 @@
 
 namespace nf {
 
-#ifndef NF_A_PREFETCH
-#define NF_A_PREFETCH 0
-#endif
-#ifndef NF_B_GROUP
-#define NF_B_GROUP 1
-#endif
-// sweep B: load face s + 1 while face s is computed (NF_B_GROUP = 1 only),
-// and load the stage's U / U_old at the start instead of after the faces
-#ifndef NF_B_PIPE
-#define NF_B_PIPE 1
-#endif
-#ifndef NF_B_PREU
-#define NF_B_PREU 1
-#endif
 // sweep A reads neighbours in the lane's own 64-cell tile from LDS
 #ifndef NF_A_LDS
 #define NF_A_LDS 1
@@ -586,8 +572,6 @@ __device__ __forceinline__ void tileA_issue(const DevMesh &m, const SweepAArgs &
     // cells.  The gathers go first: the waits for the node and state loads
     // then cover them too (masked loads cannot be counted), which measured
     // faster than issuing them after the LDS copies (1.04 vs 0.93 ms).
-    // NF_A_PREFETCH = 1 loads variable k + 1 while variable k is processed
-    // instead of all five up front (fewer registers, less parallelism)
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const uint32_t idx = t.w[s] & 0x1FFFFFFFu;
@@ -599,7 +583,7 @@ __device__ __forceinline__ void tileA_issue(const DevMesh &m, const SweepAArgs &
         for (int s = 0; s < 4; ++s)
             if (!t.loc[s]) {
 #pragma unroll
-                for (int k = 0; k < (NF_A_PREFETCH ? 1 : 5); ++k) t.Vn[s][k] = ldg(t.nsrc[s] + k * TILE);
+                for (int k = 0; k < 5; ++k) t.Vn[s][k] = ldg(t.nsrc[s] + k * TILE);
             }
     }
 }
@@ -672,11 +656,6 @@ __device__ __forceinline__ double tileA_run(const DevMesh &m, const SweepAArgs &
         // Barth factor and the face values depend on variable k only.
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            if (NF_A_PREFETCH && k + 1 < 5) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    if (!loc[s]) Vn[s][k + 1] = t.nsrc[s][(k + 1) * TILE];
-            }
             // Green-Gauss (include/Numerics.hpp:112-127): cell i adds tmp/vol,
             // cell j subtracts it, faces in reference order
             double g0 = 0.0, g1 = 0.0, g2 = 0.0;
@@ -802,8 +781,9 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const bool zero_old = st.uold_zero;
     double Up[5], Uop[5];
-    if (NF_B_PREU) {
-        // (stage 1 and a zero U_old read U twice: same lines, no branch)
+    {
+        // loaded first, outside the stage-kind branches (stage 1 and a zero
+        // U_old read U twice: same lines, no branch)
         const double *uo = (a.kind == 1 || zero_old) ? a.U : a.Uo;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
@@ -811,7 +791,6 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
             Uop[k] = uo[tix(c, k)];
         }
     }
-#if NF_B_PIPE
     // face values of both sides, one face at a time, the next face's loads
     // issued before the current face is computed
     double Vo4[4][5], Vn4[4][5];
@@ -845,47 +824,6 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
         __builtin_amdgcn_sched_barrier(0);
     }
-#else
-    // face values of both sides, loaded NF_B_GROUP faces at a time; one face
-    // at a time measured fastest at 128^3 (1.11 ms vs 1.14 for two faces and
-    // 1.20 for all four faces' loads issued up front)
-#pragma unroll
-    for (int g0 = 0; g0 < 4; g0 += NF_B_GROUP) {
-        double Vo4[NF_B_GROUP][5], Vn4[NF_B_GROUP][5];
-#pragma unroll
-        for (int q = 0; q < NF_B_GROUP; ++q) {
-            const int s = g0 + q;
-            const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
-            const bool bnd = w[s] & 0x80000000u;
-            const double *src = bnd ? a.Gf + tix(idx, 0) : a.FV + fix(idx, sn, 0);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                Vo4[q][k] = a.FV[fix(c, s, k)];
-                Vn4[q][k] = src[k * TILE];
-            }
-        }
-        if (g0 == 0) {
-            nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
-            cell_geo(m, g);
-        }
-#pragma unroll
-        for (int q = 0; q < NF_B_GROUP; ++q) {
-            const int s = g0 + q;
-            const uint32_t byte = (code >> (8 * s)) & 0xFFu;
-            V3d S;
-            face_geo(m, g, nv, byte, S, nullptr);
-            // interior faces (src/Solver.cpp:171-192), boundary faces (:197-229)
-            double UO[5], UN[5], F[5];
-            prim_to_cons(Vo4[q], UO);
-            prim_to_cons(Vn4[q], UN);
-            const uint32_t mneg = (byte & 0x80u) ? 0u : 0x80000000u;
-            rusanov_side(UO, UN, S, mneg, F);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-#endif
     if (live && a.check) {
         bool ok = true;
 #pragma unroll
@@ -899,13 +837,6 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         }
     } else {
         const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
-        if (!NF_B_PREU) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                Up[k] = a.U[tix(c, k)];
-                Uop[k] = a.kind == 1 || zero_old ? 0.0 : a.Uo[tix(c, k)];
-            }
-        }
         double Un[5], Vnew[5];
         // Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:98-132)
         if (a.kind == 1) {
