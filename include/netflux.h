@@ -67,7 +67,9 @@ typedef struct nf_params {
     int32_t time_scheme;          /* NF_EXPLICIT_EULER | NF_TVD_RK3        */
     int32_t spatial_order;        /* NF_FIRST_ORDER | NF_SECOND_ORDER      */
     int32_t limiter;              /* NF_LIMITER_*                          */
-    int32_t inv_flux_scheme;      /* NF_FLUX_RUSANOV (HLLC -> NF_E_ARG)    */
+    int32_t inv_flux_scheme;      /* NF_FLUX_RUSANOV, or NF_FLUX_HLLC with */
+                                  /* NF_PREC_FP64 (Toro's scheme: the      */
+                                  /* reference's is a stub)                */
     double CFL;
     double primvars_inf[5];       /* rho, u, v, w, p of the freestream     */
     int32_t check_physical_validity;

@@ -414,10 +414,13 @@ int run_sweepB(nf_ctx *ctx, int kind, const double *U, double *Uout, bool res, b
     // (FP64: prim(Uout) is not stored; the next sweep A evaluates it from U)
     const SweepBArgs a{ctx->FV, ctx->Gf, U, ctx->Ua, Uout, nullptr, dtc, kind, res ? ctx->res_part : nullptr,
                        ctx->prm.check_physical_validity, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+    const bool hllc = ctx->prm.inv_flux_scheme == NF_FLUX_HLLC;
     return launch_fn(ctx, "sweepB", [&] {
-        if (ctx->block == 128)
-            hipLaunchKernelGGL(k_sweepB<128>, dim3(ctx->Ncell / 128), dim3(128), 0, ctx->stream, m, a);
-        else hipLaunchKernelGGL(k_sweepB<256>, dim3(ctx->Ncell / 256), dim3(256), 0, ctx->stream, m, a);
+        const dim3 g(ctx->Ncell / ctx->block), b(ctx->block);
+        if (ctx->block == 128 && hllc) hipLaunchKernelGGL((k_sweepB<128, 1>), g, b, 0, ctx->stream, m, a);
+        else if (ctx->block == 128) hipLaunchKernelGGL((k_sweepB<128, 0>), g, b, 0, ctx->stream, m, a);
+        else if (hllc) hipLaunchKernelGGL((k_sweepB<256, 1>), g, b, 0, ctx->stream, m, a);
+        else hipLaunchKernelGGL((k_sweepB<256, 0>), g, b, 0, ctx->stream, m, a);
         return hipGetLastError();
     });
 }
@@ -727,13 +730,17 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     const nf_params &p0 = prms[0];
     for (int i = 0; i < n; ++i) {
         const nf_params &p = prms[i];
-        if (p.inv_flux_scheme != NF_FLUX_RUSANOV) return NF_E_ARG;  // HLLC: assert(false) in the reference
+        // HLLC (an assert(false) stub in the reference, src/Numerics.cpp:20-23)
+        // is the published scheme here, in the FP64 kernels only
+        if (p.inv_flux_scheme != NF_FLUX_RUSANOV && p.inv_flux_scheme != NF_FLUX_HLLC) return NF_E_ARG;
+        if (p.inv_flux_scheme == NF_FLUX_HLLC && p.precision != NF_PREC_FP64) return NF_E_ARG;
         if (p.time_scheme != NF_TVD_RK3 && p.time_scheme != NF_EXPLICIT_EULER) return NF_E_ARG;
         if (p.precision != NF_PREC_FP64 && p.precision != NF_PREC_MIXED) return NF_E_ARG;
         // one launch serves all instances: everything but the freestream is shared
         if (p.time_scheme != p0.time_scheme || p.spatial_order != p0.spatial_order || p.limiter != p0.limiter ||
             p.CFL != p0.CFL || p.check_physical_validity != p0.check_physical_validity ||
             p.rk3_zero_old_first_step != p0.rk3_zero_old_first_step || p.precision != p0.precision ||
+            p.inv_flux_scheme != p0.inv_flux_scheme ||
             p.cell_order != p0.cell_order || p.local_time_step != p0.local_time_step)
             return NF_E_ARG;
     }

@@ -79,9 +79,8 @@ def parse_input(path):
     _enum(d, "solver")
     opts = {"time_scheme": _enum(d, "time_scheme"), "inviscid_flux_scheme": _enum(d, "inviscid_flux_scheme"),
             "spatial_order": _enum(d, "spatial_order"), "initial_cond": _enum(d, "initial_cond")}
-    if opts["inviscid_flux_scheme"] == "HLLC":
-        # the reference's HLLC is an assert(false) stub (src/Numerics.cpp:20-23)
-        raise SimDirError("inviscid_flux_scheme HLLC is not implemented in the reference")
+    # HLLC is an assert(false) stub in the reference (src/Numerics.cpp:20-23);
+    # here it is Toro's published scheme (FP64)
     opts["limiter"] = _enum(d, "limiter") if opts["spatial_order"] == "Second" else "NONE"
     if opts["spatial_order"] == "Second":
         _enum(d, "grad_scheme")
@@ -110,7 +109,7 @@ def parse_input(path):
         bc[name] = t
     opts["patches"] = bc
     cfg = S.Config(time_scheme=opts["time_scheme"], spatial_order=opts["spatial_order"], limiter=opts["limiter"],
-                   CFL=float(_required(d, "CFL")),
+                   inviscid_flux_scheme=opts["inviscid_flux_scheme"], CFL=float(_required(d, "CFL")),
                    density_fs=float(d.get("density_fs", STANDARD_AIR_DENSITY)),
                    velocity_fs=tuple(float(v) for v in vel),
                    pressure_fs=float(d.get("pressure_fs", STANDARD_AIR_PRESSURE)),

@@ -121,6 +121,53 @@ void rusanov(const double *UL, const double *UR, const Vec3 &S, double *Flux) {
     for (int k = 0; k < 5; ++k) Flux[k] = area * 0.5 * ((FR[k] + FL[k]) - smax * (UR[k] - UL[k]));
 }
 
+// ---- HLLC ----
+// The reference selects HLLC by name (include/Utilities.hpp, Numerics.hpp:19-20)
+// but its body is an assert(false) stub (src/Numerics.cpp:20-23), so there is
+// no reference behaviour to restate.  This is the published HLLC solver of
+// E. F. Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics"
+// (3rd ed., 2009), section 10.4: wave speeds of Davis,
+//   S_L = min(v_L - c_L, v_R - c_R),  S_R = max(v_L + c_L, v_R + c_R),
+// contact speed (10.37)
+//   S_M = (p_R - p_L + m_L v_L - m_R v_R) / (m_L - m_R),  m_K = rho_K (S_K - v_K),
+// star states (10.39) and fluxes F*_K = F_K + S_K (U*_K - U_K) (10.38), with
+// v the velocity along n = S/|S|, scaled by |S| like the Rusanov flux.  The
+// device kernels evaluate the same operations in the same order.
+void hllc(const double *UL, const double *UR, const Vec3 &S, double *Flux) {
+    const Vec3 n = normalized(S);
+    const double area = std::sqrt(squared_norm(S));
+    const double pL = pressure(UL), pR = pressure(UR);
+    const double vL = projected_velocity(UL, n), vR = projected_velocity(UR, n);
+    const double cL = std::sqrt(GAMMA * pL / UL[0]), cR = std::sqrt(GAMMA * pR / UR[0]);
+    const double sL = std::min(vL - cL, vR - cR);
+    const double sR = std::max(vL + cL, vR + cR);
+    const double mL = UL[0] * (sL - vL), mR = UR[0] * (sR - vR);
+    const double sM = ((pR - pL) + (mL * vL - mR * vR)) / (mL - mR);
+    double F[5];
+    if (0.0 <= sL) {
+        inviscid_flux(UL, n, F);
+    } else if (sR <= 0.0) {
+        inviscid_flux(UR, n, F);
+    } else {
+        const bool left = 0.0 <= sM;
+        const double *U = left ? UL : UR;
+        const double sK = left ? sL : sR, vK = left ? vL : vR, pK = left ? pL : pR, mK = left ? mL : mR;
+        double FK[5];
+        inviscid_flux(U, n, FK);
+        const double rs = mK / (sK - sM);  // star density
+        const double dv = sM - vK;
+        const double Us[5] = {rs, rs * (U[1] / U[0] + dv * n[0]), rs * (U[2] / U[0] + dv * n[1]),
+                              rs * (U[3] / U[0] + dv * n[2]), rs * (U[4] / U[0] + dv * (sM + pK / mK))};
+        for (int k = 0; k < 5; ++k) F[k] = FK[k] + sK * (Us[k] - U[k]);
+    }
+    for (int k = 0; k < 5; ++k) Flux[k] = area * F[k];
+}
+
+void inviscid_flux_scheme(int scheme, const double *UL, const double *UR, const Vec3 &S, double *F) {
+    if (scheme == 1) hllc(UL, UR, S, F);
+    else rusanov(UL, UR, S, F);
+}
+
 // ---- boundary conditions (src/Numerics.cpp:59-171) ----
 enum BCType { NoSlipWall = 0, SlipWall = 1, FarField = 2 };
 
@@ -312,6 +359,7 @@ struct Params {
     double V_inf[5];
     int check_physical_validity;
     int rk3_zero_old_first_step;
+    int inv_flux_scheme;  // 0 Rusanov, 1 HLLC
 };
 
 // ---- solver (src/Solver.cpp, src/SolverData.cpp) ----
@@ -464,7 +512,7 @@ struct Solver {
             reconstruct(j, g.rj[f], VR);
             prim_to_cons(VL, UL);
             prim_to_cons(VR, UR);
-            rusanov(UL, UR, g.S[f], F);
+            inviscid_flux_scheme(prm.inv_flux_scheme, UL, UR, g.S[f], F);
             for (int k = 0; k < 5; ++k) {
                 R[5 * (size_t)i + k] -= F[k];
                 R[5 * (size_t)j + k] += F[k];
@@ -477,7 +525,7 @@ struct Solver {
                 bc_ghost(g.pbc[p], ff, VL, VR, g.S[f]);
                 prim_to_cons(VL, UL);
                 prim_to_cons(VR, UR);
-                rusanov(UL, UR, g.S[f], F);
+                inviscid_flux_scheme(prm.inv_flux_scheme, UL, UR, g.S[f], F);
                 for (int k = 0; k < 5; ++k) R[5 * (size_t)i + k] -= F[k];
             }
     }
@@ -552,6 +600,7 @@ struct or_params {
     double V_inf[5];
     int32_t check_physical_validity;
     int32_t rk3_zero_old_first_step;
+    int32_t inv_flux_scheme;
 };
 
 void *or_create(uint32_t n_nodes, const double *xyz, uint32_t n_cells, const uint32_t *cells,
@@ -561,7 +610,7 @@ void *or_create(uint32_t n_nodes, const double *xyz, uint32_t n_cells, const uin
         Grid g;
         build_grid(n_nodes, xyz, n_cells, cells, n_patches, patch_n, patch_tri, patch_bc, g);
         Params prm{p->time_scheme, p->spatial_order, p->limiter, p->CFL, {0, 0, 0, 0, 0},
-                   p->check_physical_validity, p->rk3_zero_old_first_step};
+                   p->check_physical_validity, p->rk3_zero_old_first_step, p->inv_flux_scheme};
         std::memcpy(prm.V_inf, p->V_inf, sizeof prm.V_inf);
         return new Solver(std::move(g), prm);
     } catch (const std::exception &e) {
@@ -671,6 +720,9 @@ void or_prim_to_cons(const double *V, double *U) { prim_to_cons(V, U); }
 void or_cons_to_prim(const double *U, double *V) { cons_to_prim(U, V); }
 void or_rusanov(const double *UL, const double *UR, const double *S, double *F) {
     rusanov(UL, UR, Vec3{{S[0], S[1], S[2]}}, F);
+}
+void or_hllc(const double *UL, const double *UR, const double *S, double *F) {
+    hllc(UL, UR, Vec3{{S[0], S[1], S[2]}}, F);
 }
 // type: 0 no-slip, 1 slip, 2 farfield with freestream V_fs
 void or_ghost(int type, const double *V_fs, const double *V_domain, const double *S, double *V_ghost) {

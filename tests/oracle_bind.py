@@ -11,7 +11,8 @@ LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 class OrParams(C.Structure):
     _fields_ = [("time_scheme", C.c_int32), ("spatial_order", C.c_int32), ("limiter", C.c_int32),
                 ("CFL", C.c_double), ("V_inf", C.c_double * 5),
-                ("check_physical_validity", C.c_int32), ("rk3_zero_old_first_step", C.c_int32)]
+                ("check_physical_validity", C.c_int32), ("rk3_zero_old_first_step", C.c_int32),
+                ("inv_flux_scheme", C.c_int32)]
 
 
 _lib = None
@@ -66,6 +67,7 @@ class Oracle:
             p.V_inf[k] = v
         p.check_physical_validity = int(config.check_physical_validity)
         p.rk3_zero_old_first_step = int(config.rk3_zero_old_first_step)
+        p.inv_flux_scheme = {"Rusanov": 0, "HLLC": 1}[config.inviscid_flux_scheme]
         err = C.create_string_buffer(256)
         self._h = lib().or_create(len(n), _p(n, C.c_double), len(c), _p(c, C.c_uint32), len(pn),
                                   _p(pn, C.c_uint32), _p(pf, C.c_uint32), _p(pb, C.c_int32), C.byref(p), err, 256)
@@ -158,6 +160,12 @@ def cons_to_prim(U):
 def rusanov(UL, UR, S):
     UL, UR, S, F = _vec(UL), _vec(UR), _vec(S), np.zeros(5)
     lib().or_rusanov(_p(UL, C.c_double), _p(UR, C.c_double), _p(S, C.c_double), _p(F, C.c_double))
+    return F
+
+
+def hllc(UL, UR, S):
+    UL, UR, S, F = _vec(UL), _vec(UR), _vec(S), np.zeros(5)
+    lib().or_hllc(_p(UL, C.c_double), _p(UR, C.c_double), _p(S, C.c_double), _p(F, C.c_double))
     return F
 
 

@@ -57,8 +57,10 @@ def test_input_defaults_and_errors(tmp_path):
     # standard-air freestream defaults (include/Utilities.hpp:100-107)
     assert (cfg.density_fs, cfg.velocity_fs, cfg.pressure_fs) == (1.225, (0.0, 0.0, 0.0), 101325.0)
     assert opts["write_stride"] == 1 and not cfg.check_physical_validity
+    cfg, _ = fe.parse_input(_yml(tmp_path, BASE.replace("Rusanov", "HLLC")))
+    assert cfg.inviscid_flux_scheme == "HLLC"
     bad = [
-        (BASE.replace("Rusanov", "HLLC"), "HLLC"),
+        (BASE.replace("Rusanov", "Roe"), "Roe"),
         (BASE.replace("limiter: Barth", "limiter: Venkat"), "Venkat"),
         (BASE.replace("CFL: 0.5\n", ""), "CFL"),
         (BASE + "  - a: FarField\n", "Duplicate patch"),

@@ -252,3 +252,39 @@ def test_primvars_export_after_graph_replays():
     b = nf.EulerSolver(cfg, mesh)
     b.set_state(a.get_solver_data().get_solution())
     np.testing.assert_array_equal(Va, b.get_solver_data().get_primvars())
+
+
+# ---- HLLC (Toro's scheme; the reference's is a stub) ----
+
+@pytest.mark.parametrize("scheme,order,limiter", [
+    ("TVD_RK3", "Second", "Barth"),
+    ("ExplicitEuler", "First", "NONE"),
+])
+def test_hllc_sod_matches_oracle(scheme, order, limiter):
+    mesh = nf.Mesh.box(10, jitter=0.1, seed=21)
+    cfg = sod_config(time_scheme=scheme, spatial_order=order, limiter=limiter, inviscid_flux_scheme="HLLC")
+    gpu, ora = _pair(mesh, cfg, sod_state(mesh))
+    gpu.eval_flux_balance()
+    ora.eval_flux_balance()
+    np.testing.assert_array_equal(gpu.get_solver_data().get_flux_balance(), ora.get("R"))
+    for _ in range(30):
+        gpu.calc_timestep(cfg)
+        gpu.step(cfg)
+    ora.run(30)
+    np.testing.assert_array_equal(gpu.get_solver_data().get_solution(), ora.get("U"))
+
+
+def test_hllc_farfield_pulse_run_graph():
+    mesh = nf.Mesh.box(8, jitter=0.15, seed=6, patches=("FarField",) * 6)
+    cfg = pulse_config(mach=0.7, inviscid_flux_scheme="HLLC")
+    gpu, ora = _pair(mesh, cfg, pulse_state(mesh, mach=0.7))
+    st = gpu.run(30, check_every=10, use_graph=True)
+    assert st["steps_done"] == 30
+    ora.run(30)
+    assert rel_l2(gpu.get_solver_data().get_solution(), ora.get("U")) <= TOL_REL_L2
+
+
+def test_hllc_needs_fp64():
+    mesh = nf.Mesh.box(4)
+    with pytest.raises(RuntimeError):
+        nf.EulerSolver(sod_config(inviscid_flux_scheme="HLLC", precision="mixed"), mesh)
