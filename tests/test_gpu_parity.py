@@ -2,7 +2,7 @@
 
 The FP64 path evaluates every formula in the oracle's operation order and
 gathers face contributions in the reference's face order, so the expected
-result is bit-identical; the north-star bar is 1e-12 relative L2 of U after
+result is bit-identical; the parity bar is 1e-12 relative L2 of U after
 100 TVD-RK3 steps, asserted on top of the exact comparison."""
 import numpy as np
 import pytest
