@@ -1391,6 +1391,8 @@ int nf_set_precision(nf_ctx *ctx, int32_t precision) {
     if (!ctx || (precision != NF_PREC_FP64 && precision != NF_PREC_MIXED)) return NF_E_ARG;
     const bool mixed = precision == NF_PREC_MIXED;
     if (mixed == ctx->mixed) return NF_OK;
+    if (mixed && ctx->prm.inv_flux_scheme == NF_FLUX_HLLC)
+        return fail(ctx, NF_E_ARG, "the HLLC flux is implemented in FP64 only");
     int rc;
     const size_t NC = ctx->Ncell;
     if (mixed && !ctx->V32) {

@@ -288,3 +288,6 @@ def test_hllc_needs_fp64():
     mesh = nf.Mesh.box(4)
     with pytest.raises(RuntimeError):
         nf.EulerSolver(sod_config(inviscid_flux_scheme="HLLC", precision="mixed"), mesh)
+    s = nf.EulerSolver(sod_config(inviscid_flux_scheme="HLLC"), mesh)
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        s.set_precision("mixed")
