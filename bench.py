@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
