@@ -55,6 +55,27 @@ def profiled_traffic(kernel, n):
     return None
 
 
+def measured_copy_gbs(mib=1024, reps=10):
+    """Device-to-device copy rate on this GPU (read + write bytes / time), the
+    practical ceiling the roofline fraction is compared with."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2 * n * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return rate
+
+
 def cpu_baseline(sample_n, steps):
     """Oracle (serial restatement of the reference solver) on one host core."""
     import netflux_amd as nf
@@ -100,6 +121,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-copy-reference", dest="copy_reference", action="store_false",
+                    help="skip timing a plain device copy beside the roofline")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
@@ -171,6 +194,10 @@ def main():
                     "kernel": dom,
                     "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
                     "algorithmic_bytes_per_launch": int(per_launch)}
+        if args.copy_reference:
+            cp = measured_copy_gbs()
+            roofline["measured_copy_GBs"] = round(cp, 1)
+            roofline["frac_of_measured_copy"] = round(achieved / cp, 4)
         step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
     else:
         roofline, step_frac = None, None  # the byte model above is the FP64 one
