@@ -55,9 +55,9 @@ def profiled_traffic(kernel, n):
     return None
 
 
-def measured_copy_gbs(mib=1024, reps=10):
+def copy_rate_gbs(mib=1024, reps=10):
     """Device-to-device copy rate on this GPU (read + write bytes / time), the
-    practical ceiling the roofline fraction is compared with."""
+    practical ceiling the roofline fraction is compared with (torch's copy)."""
     import torch
     n = mib * (1 << 20) // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -70,10 +70,17 @@ def measured_copy_gbs(mib=1024, reps=10):
         dst.copy_(src)
     e1.record()
     torch.cuda.synchronize()
-    rate = 2 * n * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
-    del src, dst
-    torch.cuda.empty_cache()
-    return rate
+    return 2 * n * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+
+
+def measured_copy_gbs():
+    """copy_rate_gbs in a child process: torch brings its own HIP runtime,
+    which cannot share a process with the one the library is linked to."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate"], capture_output=True,
+                       text=True, timeout=300)
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
 
 
 def cpu_baseline(sample_n, steps):
@@ -123,6 +130,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-copy-reference", dest="copy_reference", action="store_false",
                     help="skip timing a plain device copy beside the roofline")
+    ap.add_argument("--copy-rate", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
@@ -133,6 +141,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_rate:
+        print(json.dumps({"copy_GBs": copy_rate_gbs()}))
+        return
     if args.copy_calibration_mib:
         import torch
         n = args.copy_calibration_mib * (1 << 20) // 8
@@ -196,8 +207,9 @@ def main():
                     "algorithmic_bytes_per_launch": int(per_launch)}
         if args.copy_reference:
             cp = measured_copy_gbs()
-            roofline["measured_copy_GBs"] = round(cp, 1)
-            roofline["frac_of_measured_copy"] = round(achieved / cp, 4)
+            if cp:
+                roofline["measured_copy_GBs"] = round(cp, 1)
+                roofline["frac_of_measured_copy"] = round(achieved / cp, 4)
         step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
     else:
         roofline, step_frac = None, None  # the byte model above is the FP64 one
