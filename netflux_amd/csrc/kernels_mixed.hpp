@@ -175,13 +175,14 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     float Vc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vc[k] = a.V[tix(c, k)];
-    // neighbours in the wave's own tile come from an LDS copy of the tile;
-    // the gathers are exec-masked to the other lanes (as in the FP64 sweep)
+    // neighbours in the same block come from an LDS copy of the block's
+    // states; the gathers are exec-masked to the other lanes (as in the FP64
+    // sweep)
     float Vn[4][5];
     bool loc[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-        loc[s] = !(w[s] & 0x80000000u) && ((w[s] & 0x1FFFFFFFu) >> 6) == (p >> 6);
+        loc[s] = !(w[s] & 0x80000000u) && ((w[s] & 0x1FFFFFFFu) / B) == (p / B);
     if (ORDER != 0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
     // (its wave-level fences also order the state writes above before the
     // reads below)
     const NodeView nv = node_table_finish<B>(m, p, ntl, lds, g);
+    if (ORDER != 0) __syncthreads();  // every wave's states are in LDS
     cell_geo(m, g);
     V3f S[4], r[4];
 #pragma unroll
@@ -212,9 +214,10 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
             if (loc[s]) {
-                const uint32_t j = w[s] & 63u;
+                const uint32_t idx = w[s] & 0x1FFFFFFFu, j = idx & 63u;
+                const float *src = ldsv + ((idx >> 6) - (p & ~(uint32_t)(B - 1)) / TILE) * 5 * TILE;
 #pragma unroll
-                for (int k = 0; k < 5; ++k) Vn[s][k] = vt[k * TILE + j];
+                for (int k = 0; k < 5; ++k) Vn[s][k] = src[k * TILE + j];
             }
     }
 
