@@ -43,6 +43,10 @@ enum { NF_PREC_FP64 = 0,   /* everything FP64, bit-identical op order       */
        NF_PREC_MIXED = 1   /* FP32 flux/gradient/limiter, FP64 state/RK/norms */ };
 /* cell numbering used on the device */
 enum { NF_ORDER_INPUT = 0, NF_ORDER_RCM = 1, NF_ORDER_MORTON = 2 };
+/* element types: VTK numbering, as ElementType (include/geometry/
+ * GeometryUtilities.hpp:8-16) */
+enum { NF_ELEM_TRIANGLE = 5, NF_ELEM_QUAD = 9, NF_ELEM_TETRA = 10, NF_ELEM_HEXA = 12,
+       NF_ELEM_WEDGE = 13, NF_ELEM_PYRAMID = 14 };
 
 /* ---- primal mesh: what PrimalGrid holds after read_su2_mesh ------------
  * (src/geometry/PrimalGrid.cpp:140-244, after salome_to_vtk_connectivity).
@@ -58,6 +62,17 @@ typedef struct nf_mesh_desc {
     const uint32_t *patch_n_faces;/* [n_patches] triangles per patch       */
     const uint32_t *patch_faces;  /* [sum patch_n_faces][3]                */
     const int32_t *patch_bc;      /* [n_patches] NF_BC_*                   */
+    /* Mixed-element meshes (Elements, include/geometry/GeometryUtilities.hpp:
+     * 123-160): all four NULL for the all-tetrahedron layout above.  Else
+     * cell i has type cell_types[i] (NF_ELEM_TETRA/HEXA/WEDGE/PYRAMID) and
+     * nodes cells[cell_offsets[i] .. cell_offsets[i+1]) in VTK order, and
+     * patch face e has type patch_face_types[e] (NF_ELEM_TRIANGLE/QUAD) and
+     * nodes patch_faces[patch_face_offsets[e] .. [e+1]).  A mesh of only
+     * tetrahedra and triangles given this way is stored in the layout above. */
+    const uint8_t *cell_types;        /* [n_cells]                         */
+    const uint32_t *cell_offsets;     /* [n_cells + 1]                     */
+    const uint8_t *patch_face_types;  /* [sum patch_n_faces]               */
+    const uint32_t *patch_face_offsets; /* [sum patch_n_faces + 1]         */
 } nf_mesh_desc;
 
 /* Mirrors the Config keys read by src/ConfigParser.cpp:38-83 that the step
@@ -99,14 +114,18 @@ int nf_mesh_from_desc(const nf_mesh_desc *desc, nf_mesh **out);
 int nf_mesh_get_desc(const nf_mesh *mesh, nf_mesh_desc *desc);
 void nf_mesh_destroy(nf_mesh *mesh);
 /* SU2 mesh file (PrimalGrid::read_su2_mesh, src/geometry/PrimalGrid.cpp:
- * 140-244): tetrahedra (type 10, Salome node order, stored in VTK order by
- * swapping nodes 1 and 2 as src/geometry/GeometryUtilities.cpp:443-466
- * does) and triangle markers (type 5) only.  Marker i takes the boundary
- * type bcs[j] of names[j] == its MARKER_TAG; a marker the list does not
- * name is an error, as in the reference.  Patch order = marker order. */
+ * 140-244): tetrahedra (10), hexahedra (12), wedges (13) and pyramids (14)
+ * in Salome node order, stored in VTK order as src/geometry/
+ * GeometryUtilities.cpp:443-466 does (tetrahedron nodes 1<->2, hexahedron
+ * 1<->3 and 5<->7, pyramid 1<->3); triangle (5) and quadrilateral (9)
+ * markers as read.  Only tetrahedra and triangles give the tetrahedral
+ * layout of nf_mesh_desc, anything else the mixed-element fields.  Marker i
+ * takes the boundary type bcs[j] of names[j] == its MARKER_TAG; a marker the
+ * list does not name is an error, as in the reference.  Patch order = marker
+ * order. */
 int nf_mesh_read_su2(const char *path, int32_t n_names, const char *const *names, const int32_t *bcs,
                      nf_mesh **out);
-/* Writes tetrahedra in Salome order and one marker per patch, coordinates
+/* Writes the elements in Salome order and one marker per patch, coordinates
  * with 17 significant digits; marker tags from names (or as read, or
  * "patch_<i>"). */
 int nf_mesh_write_su2(const nf_mesh *mesh, const char *path, int32_t n_names, const char *const *names);
@@ -137,7 +156,12 @@ int nf_fv_export(const nf_mesh *mesh,
 /* ---- solver context ------------------------------------------------------
  * Replaces EulerSolver (include/Solver.hpp:52-79): constructed from the mesh
  * and config, stepped with calc_timestep + step in the Driver loop order
- * (src/Driver.cpp:31-57). */
+ * (src/Driver.cpp:31-57).  A mixed-element mesh (hexahedra, wedges,
+ * pyramids) runs on the polyhedral FP64 path: one mesh, global time step,
+ * and the calls set_freestream, set_state, calc_timestep, step,
+ * eval_flux_balance, get_state, get_field and ctx_sizes; the others return
+ * NF_E_ARG with a message, and nf_create rejects NF_PREC_MIXED and
+ * local_time_step for such a mesh. */
 typedef struct nf_ctx nf_ctx;
 
 int nf_create(const nf_mesh_desc *mesh, const nf_params *params, nf_ctx **out);

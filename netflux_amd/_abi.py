@@ -19,8 +19,10 @@ NF_BC_NOSLIP_WALL, NF_BC_SLIP_WALL, NF_BC_FARFIELD = 0, 1, 2
 NF_PREC_FP64, NF_PREC_MIXED = 0, 1
 NF_ORDER_INPUT, NF_ORDER_RCM, NF_ORDER_MORTON = 0, 1, 2
 NF_FIELD_PRIM, NF_FIELD_LIMITER, NF_FIELD_GRADIENT, NF_FIELD_RESIDUAL, NF_FIELD_STATE_OLD = range(5)
+NF_ELEM_TRIANGLE, NF_ELEM_QUAD, NF_ELEM_TETRA, NF_ELEM_HEXA, NF_ELEM_WEDGE, NF_ELEM_PYRAMID = 5, 9, 10, 12, 13, 14
 
 u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)
 f32p = C.POINTER(C.c_float)
@@ -32,6 +34,8 @@ class MeshDesc(C.Structure):
         ("n_cells", C.c_uint32), ("cells", u32p),
         ("n_patches", C.c_uint32), ("patch_n_faces", u32p),
         ("patch_faces", u32p), ("patch_bc", i32p),
+        ("cell_types", u8p), ("cell_offsets", u32p),
+        ("patch_face_types", u8p), ("patch_face_offsets", u32p),
     ]
 
 

@@ -166,6 +166,10 @@ void build_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly,
 
 // ---------------------------------------------------------------------------
 void build_fv_grid(const PrimalMesh &m, FVGrid &g) {
+    if (m.mixed()) {
+        build_fv_grid_mixed(m, g);
+        return;
+    }
     const uint32_t N = m.n_cells(), nn = m.n_nodes();
     const uint32_t P = (uint32_t)m.patch_n.size();
     if (N == 0) throw MeshError{"mesh has no cells"};

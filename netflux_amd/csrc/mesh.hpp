@@ -15,17 +15,33 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/netflux.h"
+
 namespace nf {
 
 struct PrimalMesh {
     std::vector<double> nodes;        // [n_nodes][3]
-    std::vector<uint32_t> cells;      // [n_cells][4] VTK order
+    std::vector<uint32_t> cells;      // [n_cells][4] VTK order (mixed: element nodes, see coff)
     std::vector<uint32_t> patch_n;    // [P]
-    std::vector<uint32_t> patch_tri;  // [sum][3]
+    std::vector<uint32_t> patch_tri;  // [sum][3] (mixed: face nodes, see foff)
     std::vector<int32_t> patch_bc;    // [P]
+    // Mixed-element meshes only; empty when every cell is a tetrahedron and
+    // every patch face a triangle.  ctype = VTK type, coff/foff = offsets
+    // into cells/patch_tri ([n+1]).
+    std::vector<uint8_t> ctype, ftype;
+    std::vector<uint32_t> coff, foff;
+    bool mixed() const { return !ctype.empty(); }
     uint32_t n_nodes() const { return (uint32_t)(nodes.size() / 3); }
-    uint32_t n_cells() const { return (uint32_t)(cells.size() / 4); }
+    uint32_t n_cells() const { return mixed() ? (uint32_t)ctype.size() : (uint32_t)(cells.size() / 4); }
 };
+
+// Element tables of the reference (include/geometry/GeometryUtilities.hpp:
+// 8-32, src/geometry/GeometryUtilities.cpp:48-240), VTK numbering.
+enum : int { ELEM_TRI = 5, ELEM_QUAD = 9, ELEM_TET = 10, ELEM_HEX = 12, ELEM_WEDGE = 13, ELEM_PYRAMID = 14 };
+int elem_n_nodes(int type);  // 0 for an unknown type
+int elem_n_faces(int type);  // volume elements
+// face k of a volume element: its node count (3 or 4) and nodes
+int elem_face(int type, const uint32_t *ve, int k, uint32_t fe[4]);
 
 struct FVGrid {
     uint32_t N = 0, NT = 0, Fint = 0, F = 0;
@@ -78,8 +94,14 @@ void build_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly,
                double lz, double jitter, uint64_t seed, double bump_h,
                double bump_sigma, const int32_t bc[6], PrimalMesh &out);
 
-// Throws MeshError on inconsistent input.
+// Throws MeshError on inconsistent input.  A mixed-element mesh fills
+// everything but fnodes and flip (which only the tetrahedral layout uses).
 void build_fv_grid(const PrimalMesh &m, FVGrid &g);
+void build_fv_grid_mixed(const PrimalMesh &m, FVGrid &g);
+
+// Primal mesh from the C descriptor (either layout); a mixed description of
+// an all-tetrahedron mesh is stored in the tetrahedral layout.
+void primal_from_desc(const nf_mesh_desc &d, PrimalMesh &m);
 
 // order: 0 = input order, 1 = reverse Cuthill-McKee, 2 = Morton (Z-order of centroids).
 void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order,

@@ -23,7 +23,8 @@ int ensure_fv(nf_mesh *m) {
     if (m->fv_built) return NF_OK;
     try {
         nf::build_fv_grid(m->pm, m->fv);
-    } catch (const nf::MeshError &) {
+    } catch (const nf::MeshError &e) {
+        g_mesh_err = e.msg;
         return NF_E_MESH;
     }
     m->fv_built = true;
@@ -59,19 +60,20 @@ int nf_mesh_from_desc(const nf_mesh_desc *d, nf_mesh **out) {
     *out = nullptr;
     nf_mesh *m = new (std::nothrow) nf_mesh;
     if (!m) return NF_E_ARG;
-    m->pm.nodes.assign(d->nodes, d->nodes + 3 * (size_t)d->n_nodes);
-    m->pm.cells.assign(d->cells, d->cells + 4 * (size_t)d->n_cells);
-    m->pm.patch_n.assign(d->patch_n_faces, d->patch_n_faces + d->n_patches);
-    size_t nt = 0;
-    for (uint32_t p = 0; p < d->n_patches; ++p) nt += d->patch_n_faces[p];
-    m->pm.patch_tri.assign(d->patch_faces, d->patch_faces + 3 * nt);
-    m->pm.patch_bc.assign(d->patch_bc, d->patch_bc + d->n_patches);
+    try {
+        nf::primal_from_desc(*d, m->pm);
+    } catch (const nf::MeshError &e) {
+        g_mesh_err = e.msg;
+        delete m;
+        return NF_E_MESH;
+    }
     *out = m;
     return NF_OK;
 }
 
 int nf_mesh_get_desc(const nf_mesh *m, nf_mesh_desc *d) {
     if (!m || !d) return NF_E_ARG;
+    std::memset(d, 0, sizeof *d);
     d->n_nodes = m->pm.n_nodes();
     d->nodes = m->pm.nodes.data();
     d->n_cells = m->pm.n_cells();
@@ -80,6 +82,12 @@ int nf_mesh_get_desc(const nf_mesh *m, nf_mesh_desc *d) {
     d->patch_n_faces = m->pm.patch_n.data();
     d->patch_faces = m->pm.patch_tri.data();
     d->patch_bc = m->pm.patch_bc.data();
+    if (m->pm.mixed()) {
+        d->cell_types = m->pm.ctype.data();
+        d->cell_offsets = m->pm.coff.data();
+        d->patch_face_types = m->pm.ftype.data();
+        d->patch_face_offsets = m->pm.foff.data();
+    }
     return NF_OK;
 }
 

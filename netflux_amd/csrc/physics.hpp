@@ -184,4 +184,68 @@ NF_HD void ghost_value(int bc, const T *Vd, Vec3T<T> S, const FarField<T> &fs, T
     }
 }
 
+#ifdef __HIPCC__
+// HLLC flux (Toro, "Riemann Solvers and Numerical Methods for Fluid
+// Dynamics", 3rd ed., 2009, section 10.4; the reference's HLLC is an
+// assert(false) stub, src/Numerics.cpp:20-23): Davis wave speeds, contact
+// speed (10.37), star states (10.39), F*_K = F_K + S_K (U*_K - U_K) (10.38),
+// scaled by |S|.  The oracle's hllc in the same operations and order, with
+// plain (correctly rounded) divisions, so both give the same bits; UL is the
+// state of the reference's cell i, UR of cell j (or the ghost), S = S_ij.
+__device__ __forceinline__ double pressure_d(const double *U) {
+    return Consts<double>::GM1 * (U[4] - 0.5 / U[0] * ((U[1] * U[1] + U[2] * U[2]) + U[3] * U[3]));
+}
+__device__ __forceinline__ void inviscid_flux_d(const double *U, Vec3T<double> n, double *F) {
+    const double rho = U[0], p = pressure_d(U);
+    const double vn = ((U[1] * n.x + U[2] * n.y) + U[3] * n.z) / rho;
+    F[0] = vn * rho;
+    F[1] = vn * U[1] + p * n.x;
+    F[2] = vn * U[2] + p * n.y;
+    F[3] = vn * U[3] + p * n.z;
+    F[4] = vn * (U[4] + p);
+}
+__device__ __forceinline__ void hllc_d(const double *UL, const double *UR, Vec3T<double> S, double *Flux) {
+    const double z = (S.x * S.x + S.y * S.y) + S.z * S.z;
+    Vec3T<double> n = S;
+    if (z > 0.0) {
+        const double r = sqrt(z);
+        n = {S.x / r, S.y / r, S.z / r};
+    }
+    const double area = sqrt(z);
+    const double G = Consts<double>::GAMMA;
+    const double pL = pressure_d(UL), pR = pressure_d(UR);
+    const double vL = ((UL[1] * n.x + UL[2] * n.y) + UL[3] * n.z) / UL[0];
+    const double vR = ((UR[1] * n.x + UR[2] * n.y) + UR[3] * n.z) / UR[0];
+    const double cL = sqrt(G * pL / UL[0]), cR = sqrt(G * pR / UR[0]);
+    const double aL = vL - cL, bL = vR - cR, aR = vL + cL, bR = vR + cR;
+    const double sL = (bL < aL) ? bL : aL;  // std::min / std::max
+    const double sR = (aR < bR) ? bR : aR;
+    const double mL = UL[0] * (sL - vL), mR = UR[0] * (sR - vR);
+    const double sM = ((pR - pL) + (mL * vL - mR * vR)) / (mL - mR);
+    double F[5];
+    if (0.0 <= sL) {
+        inviscid_flux_d(UL, n, F);
+    } else if (sR <= 0.0) {
+        inviscid_flux_d(UR, n, F);
+    } else {
+        const bool left = 0.0 <= sM;
+        double U[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = left ? UL[k] : UR[k];
+        const double sK = left ? sL : sR, vK = left ? vL : vR, pK = left ? pL : pR, mK = left ? mL : mR;
+        double FK[5];
+        inviscid_flux_d(U, n, FK);
+        const double rs = mK / (sK - sM);  // star density
+        const double dv = sM - vK;
+        const double Us[5] = {rs, rs * (U[1] / U[0] + dv * n.x), rs * (U[2] / U[0] + dv * n.y),
+                              rs * (U[3] / U[0] + dv * n.z), rs * (U[4] / U[0] + dv * (sM + pK / mK))};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) F[k] = FK[k] + sK * (Us[k] - U[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Flux[k] = area * F[k];
+}
+
+#endif
+
 }  // namespace nf

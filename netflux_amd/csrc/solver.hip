@@ -21,6 +21,7 @@
 #include "kernels_mixed.hpp"
 #include "mesh.hpp"
 #include "mlp.hpp"
+#include "poly.hpp"
 
 using namespace nf;
 
@@ -252,6 +253,7 @@ struct nf_ctx {
     bool timing = false;
     std::map<std::string, std::pair<double, int>> ktime;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    nf::PolyPath *poly = nullptr;  // mixed-element mesh: the polyhedral path (n_inst = 0)
 };
 
 namespace {
@@ -746,6 +748,39 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NF_E_NOGPU;
+    // a mixed-element mesh (hexahedra, wedges, pyramids) takes the polyhedral path
+    for (int i = 0; i < n; ++i) {
+        if (!descs[i].cell_types) continue;
+        nf::PrimalMesh pm;
+        try {
+            nf::primal_from_desc(descs[i], pm);
+        } catch (const nf::MeshError &) {
+            return NF_E_MESH;
+        }
+        if (!pm.mixed()) continue;
+        if (n != 1) return NF_E_ARG;
+        for (int32_t b : pm.patch_bc)
+            if (b < 0 || b > 2) return NF_E_ARG;
+        nf::FVGrid g;
+        try {
+            nf::build_fv_grid(pm, g);
+        } catch (const nf::MeshError &) {
+            return NF_E_MESH;
+        }
+        std::unique_ptr<nf_ctx> hold(new nf_ctx);
+        nf_ctx *ctx = hold.get();
+        ctx->prm = p0;
+        ctx->n_inst = 0;
+        HIPC(hipGetDevice(&ctx->device));
+        HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        const int rc = nf::poly_create(pm, g, p0, ctx->stream, &ctx->poly, ctx->err);
+        if (rc) {
+            (void)hipStreamDestroy(ctx->stream);
+            return rc;
+        }
+        *out = hold.release();
+        return NF_OK;
+    }
     std::unique_ptr<nf_ctx> hold(new nf_ctx);
     nf_ctx *ctx = hold.get();
     ctx->prm = p0;
@@ -939,8 +974,17 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     return NF_OK;
 }
 
-bool single(const nf_ctx *ctx) { return ctx && ctx->n_inst == 1; }
-bool valid_inst(const nf_ctx *ctx, int32_t i) { return ctx && i >= 0 && i < ctx->n_inst; }
+const char *POLY_UNSUPPORTED =
+    "not available for a mixed-element mesh (supported: set_freestream, set_state, calc_timestep, step, "
+    "eval_flux_balance, get_state, get_field, ctx_sizes)";
+bool single(const nf_ctx *ctx) {
+    if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
+    return ctx && ctx->n_inst == 1;
+}
+bool valid_inst(const nf_ctx *ctx, int32_t i) {
+    if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
+    return ctx && i >= 0 && i < ctx->n_inst;
+}
 
 // one step of a single-instance context with the dt held on the device
 int step_device_dt(nf_ctx *ctx) {
@@ -1142,6 +1186,7 @@ int nf_create_batch(int32_t n, const nf_mesh_desc *meshes, const nf_params *para
 int nf_destroy(nf_ctx *ctx) {
     if (!ctx) return NF_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    nf::poly_destroy(ctx->poly);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (void *p : ctx->owned) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1153,6 +1198,10 @@ const char *nf_last_error(const nf_ctx *ctx) { return ctx ? ctx->err.c_str() : "
 
 int nf_ctx_sizes(const nf_ctx *ctx, nf_fv_sizes *s) {
     if (!ctx || !s) return NF_E_ARG;
+    if (ctx->poly) {
+        *s = nf::poly_sizes(ctx->poly);
+        return NF_OK;
+    }
     *s = ctx->inst[0].sizes;
     return NF_OK;
 }
@@ -1171,6 +1220,7 @@ int nf_inst_sizes(const nf_ctx *ctx, int32_t i, nf_fv_sizes *s) {
 
 int nf_set_freestream(nf_ctx *ctx) {
     if (!ctx) return NF_E_ARG;
+    if (ctx->poly) return nf::poly_set_freestream(ctx->poly, ctx->err);
     for (int i = 0; i < ctx->n_inst; ++i) {
         int rc = set_freestream(ctx, i);
         if (rc) return rc;
@@ -1179,6 +1229,7 @@ int nf_set_freestream(nf_ctx *ctx) {
 }
 
 int nf_set_state(nf_ctx *ctx, const double *U, const double *U_old) {
+    if (ctx && ctx->poly) return U ? nf::poly_set_state(ctx->poly, U, U_old, ctx->err) : NF_E_ARG;
     if (!single(ctx) || !U) return NF_E_ARG;
     return set_state(ctx, 0, U, U_old);
 }
@@ -1189,6 +1240,7 @@ int nf_inst_set_state(nf_ctx *ctx, int32_t i, const double *U, const double *U_o
 }
 
 int nf_calc_timestep(nf_ctx *ctx, double *dt_out) {
+    if (ctx && ctx->poly) return nf::poly_calc_timestep(ctx->poly, dt_out, ctx->err);
     if (!single(ctx)) return NF_E_ARG;
     int rc;
     if (!ctx->inst[0].dt_valid) {
@@ -1209,6 +1261,7 @@ int nf_calc_timestep(nf_ctx *ctx, double *dt_out) {
 }
 
 int nf_step(nf_ctx *ctx, double dt) {
+    if (ctx && ctx->poly) return nf::poly_step(ctx->poly, dt, ctx->err);
     if (!single(ctx)) return NF_E_ARG;
     hipLaunchKernelGGL(k_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->st, 1, 1, 0, dt);
     HIPC(hipGetLastError());
@@ -1221,6 +1274,7 @@ int nf_run(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *st) {
 }
 
 int nf_run_batch(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *stats) {
+    if (ctx && ctx->poly) return fail(ctx, NF_E_ARG, POLY_UNSUPPORTED);
     if (!ctx) return NF_E_ARG;
     return run_loop(ctx, o, stats);
 }
@@ -1244,6 +1298,7 @@ int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
 }
 
 int nf_get_state(nf_ctx *ctx, double *U) {
+    if (ctx && ctx->poly) return U ? nf::poly_get_state(ctx->poly, U, ctx->err) : NF_E_ARG;
     if (!single(ctx) || !U) return NF_E_ARG;
     return download_state(ctx, 0, (const double *)ctx->Ua, U);
 }
@@ -1257,6 +1312,7 @@ int nf_inst_get_state(nf_ctx *ctx, int32_t i, double *U) {
 // evaluate_flux_balance, src/Solver.cpp:54-81) without updating U, keeping
 // gradient, limiter and R for nf_get_field.
 int nf_eval_flux_balance(nf_ctx *ctx) {
+    if (ctx && ctx->poly) return nf::poly_eval_flux_balance(ctx->poly, ctx->err);
     if (!single(ctx)) return NF_E_ARG;
     int rc;
     const size_t NC = ctx->Ncell;
@@ -1279,6 +1335,7 @@ int nf_eval_flux_balance(nf_ctx *ctx) {
 }
 
 int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
+    if (ctx && ctx->poly) return out ? nf::poly_get_field(ctx->poly, which, out, ctx->err) : NF_E_ARG;
     if (!single(ctx) || !out) return NF_E_ARG;
     switch (which) {
     case NF_FIELD_PRIM: {
@@ -1349,6 +1406,7 @@ int nf_inst_set_residual_ref(nf_ctx *ctx, int32_t i, double ref) {
 }
 
 int nf_eval_residuals(nf_ctx *ctx, double *out) {
+    if (ctx && ctx->poly) return fail(ctx, NF_E_ARG, POLY_UNSUPPORTED);
     if (!ctx || !out) return NF_E_ARG;
     return eval_residuals(ctx, out);
 }
@@ -1388,6 +1446,7 @@ int nf_inst_warm_start(nf_ctx *ctx, int32_t i, nf_mlp *net, const double *cp, in
 }
 
 int nf_set_precision(nf_ctx *ctx, int32_t precision) {
+    if (ctx && ctx->poly) return fail(ctx, NF_E_ARG, POLY_UNSUPPORTED);
     if (!ctx || (precision != NF_PREC_FP64 && precision != NF_PREC_MIXED)) return NF_E_ARG;
     const bool mixed = precision == NF_PREC_MIXED;
     if (mixed == ctx->mixed) return NF_OK;
@@ -1425,6 +1484,7 @@ int nf_set_precision(nf_ctx *ctx, int32_t precision) {
 }
 
 int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap) {
+    if (ctx && ctx->poly) return fail(ctx, NF_E_ARG, POLY_UNSUPPORTED);
     if (!ctx || (block != 128 && block != 256)) return NF_E_ARG;
     ctx->block = block;
     ctx->xcd_remap = xcd_remap ? 1 : 0;
@@ -1436,6 +1496,7 @@ int nf_set_launch(nf_ctx *ctx, int32_t block, int32_t xcd_remap) {
 // and restores the state, run counters and dt afterwards.  Results do not
 // depend on the configuration.
 int nf_autotune(nf_ctx *ctx, int32_t steps, nf_tune_result *out) {
+    if (ctx && ctx->poly) return fail(ctx, NF_E_ARG, POLY_UNSUPPORTED);
     if (!ctx || steps < 1) return NF_E_ARG;
     int rc;
     Snapshot snap;

@@ -1,18 +1,21 @@
-// SU2 mesh files for the tetrahedral path: reader with the reference's
-// conventions (PrimalGrid::read_su2_mesh, src/geometry/PrimalGrid.cpp:140-244)
-// and a writer for round trips and generated test cases.  CPU only.
+// SU2 mesh files: reader with the reference's conventions
+// (PrimalGrid::read_su2_mesh, src/geometry/PrimalGrid.cpp:140-244) and a
+// writer for round trips and generated test cases.  CPU only.
 //
 //   NDIME= 3
-//   NELEM= n            then n lines: 10 a b c d [index]   (Salome node order)
+//   NELEM= n            then n lines: type nodes... [index]   (Salome node order)
 //   NPOIN= n            then n lines: x y z [index]
 //   NMARK= m            then per marker: MARKER_TAG= name, MARKER_ELEMS= k,
-//                       k lines: 5 a b c
+//                       k lines: type nodes...
 //
-// Tetrahedra arrive in Salome order and are stored in VTK order by swapping
-// nodes 1 and 2 (salome_to_vtk_connectivity, src/geometry/
-// GeometryUtilities.cpp:443-466); triangles are stored as read.  Only
-// tetrahedra and triangles are accepted: the device path is tetrahedral.
+// Volume elements: tetrahedra (10), hexahedra (12), wedges (13), pyramids
+// (14); marker elements: triangles (5), quadrilaterals (9).  Elements arrive
+// in Salome order and are stored in VTK order (salome_to_vtk_connectivity,
+// src/geometry/GeometryUtilities.cpp:443-466: tetrahedron nodes 1<->2,
+// hexahedron 1<->3 and 5<->7, pyramid 1<->3); marker elements are stored as
+// read.  A file of only tetrahedra and triangles gives the tetrahedral layout.
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <new>
@@ -26,7 +29,6 @@
 namespace nf {
 
 namespace {
-constexpr int VTK_TRIANGLE = 5, VTK_TETRA = 10;
 
 // next meaningful token stream line: skips blank lines and '%' comments
 bool next_line(std::istream &is, std::istringstream &ls) {
@@ -62,28 +64,41 @@ T keyword(std::istream &is, const char *key) {
 }
 }  // namespace
 
+// Salome <-> VTK node order (the swaps are their own inverse)
+static void salome_vtk(int t, uint32_t *n) {
+    if (t == ELEM_TET) std::swap(n[1], n[2]);
+    else if (t == ELEM_HEX) std::swap(n[1], n[3]), std::swap(n[5], n[7]);
+    else if (t == ELEM_PYRAMID) std::swap(n[1], n[3]);
+}
+
 void read_su2(const std::string &path, PrimalMesh &m, std::vector<std::string> &names) {
     std::ifstream is(path);
     if (!is) throw MeshError("Couldn't open the mesh file: " + path);
     if (keyword<int>(is, "NDIME=") != 3) throw MeshError("Only three dimensions (NDIME=3) is accepted");
     const long ne = keyword<long>(is, "NELEM=");
     if (ne <= 0) throw MeshError("NELEM must be positive");
-    m.cells.resize(4 * (size_t)ne);
+    m = PrimalMesh{};
+    m.coff.push_back(0);
     std::istringstream ls;
+    bool all_tet = true;
     for (long e = 0; e < ne; ++e) {
         if (!next_line(is, ls)) throw MeshError("unexpected end of element list");
-        int t = 0;
+        int t = -1;
         ls >> t;
-        if (t != VTK_TETRA)
-            throw MeshError("element type " + std::to_string(t) +
-                            " in the volume mesh: only tetrahedra (10) are supported");
-        uint32_t n[4];
-        if (!(ls >> n[0] >> n[1] >> n[2] >> n[3])) throw MeshError("bad tetrahedron line");
-        // Salome -> VTK
-        m.cells[4 * e] = n[0];
-        m.cells[4 * e + 1] = n[2];
-        m.cells[4 * e + 2] = n[1];
-        m.cells[4 * e + 3] = n[3];
+        const int nn = elem_n_nodes(t);
+        if (nn == 0)
+            throw MeshError("Illegal su2 volume element identifier (" + std::to_string(t) +
+                            ") detected in su2 mesh file");
+        if (elem_n_faces(t) == 0)
+            throw MeshError("Face element of type " + std::to_string(t) + " detected in su2 mesh file in the volume mesh");
+        uint32_t n[8];
+        for (int q = 0; q < nn; ++q)
+            if (!(ls >> n[q])) throw MeshError("bad element line");
+        salome_vtk(t, n);
+        m.cells.insert(m.cells.end(), n, n + nn);
+        m.ctype.push_back((uint8_t)t);
+        m.coff.push_back((uint32_t)m.cells.size());
+        all_tet = all_tet && t == ELEM_TET;
     }
     const long np = keyword<long>(is, "NPOIN=");
     if (np <= 0) throw MeshError("NPOIN must be positive");
@@ -95,9 +110,8 @@ void read_su2(const std::string &path, PrimalMesh &m, std::vector<std::string> &
     for (uint32_t v : m.cells)
         if (v >= (uint64_t)np) throw MeshError("element node index out of range");
     const long nm = keyword<long>(is, "NMARK=");
-    m.patch_n.clear();
-    m.patch_tri.clear();
     names.clear();
+    m.foff.push_back(0);
     for (long q = 0; q < nm; ++q) {
         std::istringstream tl;
         if (!next_line(is, tl)) throw MeshError("missing MARKER_TAG=");
@@ -110,20 +124,28 @@ void read_su2(const std::string &path, PrimalMesh &m, std::vector<std::string> &
         const long nf = keyword<long>(is, "MARKER_ELEMS=");
         for (long f = 0; f < nf; ++f) {
             if (!next_line(is, ls)) throw MeshError("unexpected end of marker " + name);
-            int t = 0;
+            int t = -1;
             ls >> t;
-            if (t != VTK_TRIANGLE)
-                throw MeshError("marker element type " + std::to_string(t) + " in marker " + name +
-                                ": only triangles (5) are supported");
-            uint32_t a, b, c;
-            if (!(ls >> a >> b >> c)) throw MeshError("bad marker element line");
-            if (a >= (uint64_t)np || b >= (uint64_t)np || c >= (uint64_t)np)
-                throw MeshError("marker node index out of range");
-            m.patch_tri.insert(m.patch_tri.end(), {a, b, c});
+            const int nn = elem_n_nodes(t);
+            if (nn == 0)
+                throw MeshError("Illegal su2 element identifier (" + std::to_string(t) + ") detected in su2 mesh file");
+            if (elem_n_faces(t) != 0)
+                throw MeshError("Volume element of type " + std::to_string(t) +
+                                " detected in su2 mesh file among marker elements (marker " + name + ")");
+            uint32_t n[4];
+            for (int r = 0; r < nn; ++r)
+                if (!(ls >> n[r])) throw MeshError("bad marker element line");
+            for (int r = 0; r < nn; ++r)
+                if (n[r] >= (uint64_t)np) throw MeshError("marker node index out of range");
+            m.patch_tri.insert(m.patch_tri.end(), n, n + nn);
+            m.ftype.push_back((uint8_t)t);
+            m.foff.push_back((uint32_t)m.patch_tri.size());
+            all_tet = all_tet && t == ELEM_TRI;
         }
         m.patch_n.push_back((uint32_t)nf);
         names.push_back(name);
     }
+    if (all_tet) m.ctype.clear(), m.coff.clear(), m.ftype.clear(), m.foff.clear();
 }
 
 void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<std::string> &names) {
@@ -131,8 +153,15 @@ void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<s
     if (!f) throw MeshError("Couldn't open " + path + " for writing");
     std::fprintf(f, "NDIME= 3\nNELEM= %u\n", m.n_cells());
     for (uint32_t e = 0; e < m.n_cells(); ++e) {
-        const uint32_t *n = &m.cells[4 * (size_t)e];
-        std::fprintf(f, "%d %u %u %u %u %u\n", VTK_TETRA, n[0], n[2], n[1], n[3], e);  // VTK -> Salome
+        const int t = m.mixed() ? m.ctype[e] : ELEM_TET;
+        const uint32_t *src = m.mixed() ? &m.cells[m.coff[e]] : &m.cells[4 * (size_t)e];
+        uint32_t n[8];
+        const int nn = elem_n_nodes(t);
+        std::copy(src, src + nn, n);
+        salome_vtk(t, n);  // VTK -> Salome
+        std::fprintf(f, "%d", t);
+        for (int q = 0; q < nn; ++q) std::fprintf(f, " %u", n[q]);
+        std::fprintf(f, " %u\n", e);
     }
     std::fprintf(f, "NPOIN= %u\n", m.n_nodes());
     for (uint32_t i = 0; i < m.n_nodes(); ++i)
@@ -142,9 +171,13 @@ void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<s
     size_t t = 0;
     for (size_t q = 0; q < m.patch_n.size(); ++q) {
         std::fprintf(f, "MARKER_TAG= %s\nMARKER_ELEMS= %u\n", names[q].c_str(), m.patch_n[q]);
-        for (uint32_t k = 0; k < m.patch_n[q]; ++k, ++t)
-            std::fprintf(f, "%d %u %u %u\n", VTK_TRIANGLE, m.patch_tri[3 * t], m.patch_tri[3 * t + 1],
-                         m.patch_tri[3 * t + 2]);
+        for (uint32_t k = 0; k < m.patch_n[q]; ++k, ++t) {
+            const int ty = m.mixed() ? m.ftype[t] : ELEM_TRI;
+            const uint32_t *src = m.mixed() ? &m.patch_tri[m.foff[t]] : &m.patch_tri[3 * t];
+            std::fprintf(f, "%d", ty);
+            for (int r = 0; r < elem_n_nodes(ty); ++r) std::fprintf(f, " %u", src[r]);
+            std::fprintf(f, "\n");
+        }
     }
     if (std::fclose(f) != 0) throw MeshError("write error on " + path);
 }

@@ -137,13 +137,28 @@ def grid_validity_report(mesh):
     volume = (n1-n0) x (n2-n0) . (n3-n0) / 6 in VTK order).  The reference
     constructs these errors without throwing them, so a run continues; here
     they are returned and printed as warnings."""
-    nodes, cells, *_ = mesh.arrays()
-    P = nodes[cells.astype(np.int64)]                       # [N][4][3]
-    a, b, c = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]
-    vol = np.einsum("ij,ij->i", np.cross(a, b), c) / 6.0
-    cen = P.mean(axis=1)
-    hi = np.maximum(P.max(axis=1), cen)
-    lo = np.minimum(P.min(axis=1), cen)
+    if mesh.is_mixed:
+        # element volumes and centroids by the reference's formulas
+        # (volume_element_calc_geometry_properties), from the face graph
+        ct, co, cn, *_ = mesh.elements()
+        d = mesh.desc
+        nodes = np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes, 3))
+        g = mesh.fv_grid()
+        N = len(ct)
+        vol, cen = g["volume"][:N], g["centroid"][:N]
+        hi, lo = cen.copy(), cen.copy()
+        for e in range(N):
+            P = nodes[cn[co[e]:co[e + 1]].astype(np.int64)]
+            hi[e] = np.maximum(hi[e], P.max(axis=0))
+            lo[e] = np.minimum(lo[e], P.min(axis=0))
+    else:
+        nodes, cells, *_ = mesh.arrays()
+        P = nodes[cells.astype(np.int64)]                       # [N][4][3]
+        a, b, c = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]
+        vol = np.einsum("ij,ij->i", np.cross(a, b), c) / 6.0
+        cen = P.mean(axis=1)
+        hi = np.maximum(P.max(axis=1), cen)
+        lo = np.minimum(P.min(axis=1), cen)
     flat = np.any(np.isclose(hi, cen, rtol=1e-12, atol=0.0) | np.isclose(lo, cen, rtol=1e-12, atol=0.0), axis=1)
     return {"negative_volume": np.nonzero(vol < 0.0)[0], "centroid_on_bbox": np.nonzero(flat)[0]}
 
@@ -153,11 +168,13 @@ def _fmt(full):
 
 
 def write_vtk(path, mesh, U, full_precision=False, limiter=None, gradient=None):
-    """Legacy ASCII VTK of the tetrahedral mesh with cell density, velocity
-    and pressure (src/Output.cpp:48-126)."""
-    nodes, cells, *_ = mesh.arrays()
+    """Legacy ASCII VTK of the mesh (tetrahedra or mixed elements, VTK cell
+    types) with cell density, velocity and pressure (src/Output.cpp:48-126)."""
+    ct, co, cn, *_ = mesh.elements()
+    d = mesh.desc
+    nodes = np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes, 3))
     f = _fmt(full_precision)
-    N = len(cells)
+    N = len(ct)
     rho = U[:, 0]
     vel = U[:, 1:4] / U[:, :1]
     p = (1.4 - 1.0) * (U[:, 4] - 0.5 * ((U[:, 1] * U[:, 1] + U[:, 2] * U[:, 2]) + U[:, 3] * U[:, 3]) / U[:, 0])
@@ -165,10 +182,15 @@ def write_vtk(path, mesh, U, full_precision=False, limiter=None, gradient=None):
         o.write("# vtk DataFile Version 3.0\nNetFlux\nASCII\n\nDATASET UNSTRUCTURED_GRID\n\n")
         o.write(f"POINTS {len(nodes)} double\n")
         np.savetxt(o, nodes, fmt=f)
-        o.write(f"\nCELLS {N} {N * 5}\n")
-        np.savetxt(o, np.column_stack([np.full(N, 4), cells]), fmt="%d")
+        o.write(f"\nCELLS {N} {N + len(cn)}\n")
+        if mesh.is_mixed:
+            for e in range(N):
+                row = cn[co[e]:co[e + 1]]
+                o.write(f"{len(row)} " + " ".join(str(int(x)) for x in row) + "\n")
+        else:
+            np.savetxt(o, np.column_stack([np.full(N, 4), cn.reshape(N, 4)]), fmt="%d")
         o.write(f"\nCELL_TYPES {N}\n")
-        np.savetxt(o, np.full(N, 10), fmt="%d")
+        np.savetxt(o, ct.astype(np.int64), fmt="%d")
         o.write(f"\nCELL_DATA {N}\n\nSCALARS density double 1\nLOOKUP_TABLE default\n")
         np.savetxt(o, rho, fmt=f)
         o.write("\nVECTORS velocity double\n")

@@ -90,7 +90,8 @@ class Config:
 
 
 class Mesh:
-    """Primal tetrahedral mesh (what PrimalGrid holds), owned by the library."""
+    """Primal mesh (what PrimalGrid holds), owned by the library: tetrahedra,
+    or mixed elements (hexahedra, wedges, pyramids, tetrahedra)."""
 
     def __init__(self, handle):
         self._h = handle
@@ -158,6 +159,59 @@ class Mesh:
         _check(A.lib().nf_mesh_from_desc(C.byref(d), C.byref(h)), what="nf_mesh_from_desc")
         return cls(h)
 
+    @classmethod
+    def from_elements(cls, nodes, cells, patch_faces, patch_bc):
+        """Mixed-element mesh (the reference's Elements): cells is a list of
+        node lists in VTK order whose length gives the type (4 tetrahedron,
+        5 pyramid, 6 wedge, 8 hexahedron); patch_faces holds per patch a list
+        of node lists (3 triangle, 4 quadrilateral).  An all-tetrahedron input
+        gives the tetrahedral layout."""
+        ctype_of = {4: A.NF_ELEM_TETRA, 5: A.NF_ELEM_PYRAMID, 6: A.NF_ELEM_WEDGE, 8: A.NF_ELEM_HEXA}
+        ftype_of = {3: A.NF_ELEM_TRIANGLE, 4: A.NF_ELEM_QUAD}
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        ct = np.array([ctype_of[len(c)] for c in cells], dtype=np.uint8)
+        co = np.concatenate([[0], np.cumsum([len(c) for c in cells])]).astype(np.uint32)
+        cn = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.uint32) for c in cells]), dtype=np.uint32)
+        faces = [f for p in patch_faces for f in p]
+        pn = np.array([len(p) for p in patch_faces], dtype=np.uint32)
+        ft = np.array([ftype_of[len(f)] for f in faces], dtype=np.uint8)
+        fo = np.concatenate([[0], np.cumsum([len(f) for f in faces])]).astype(np.uint32)
+        fn = np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=np.uint32) for f in faces]), dtype=np.uint32)
+        pbc = np.array([BC_NAMES[b] if isinstance(b, str) else int(b) for b in patch_bc], dtype=np.int32)
+        d = A.MeshDesc(len(nodes), _f64(nodes), len(cells), _u32(cn), len(pn), _u32(pn), _u32(fn), _i32(pbc),
+                       ct.ctypes.data_as(A.u8p), _u32(co), ft.ctypes.data_as(A.u8p), _u32(fo))
+        h = C.c_void_p()
+        rc = A.lib().nf_mesh_from_desc(C.byref(d), C.byref(h))
+        if rc != A.NF_OK:
+            raise RuntimeError(f"nf_mesh_from_desc failed ({rc}): {A.lib().nf_mesh_last_error().decode()}")
+        return cls(h)
+
+    @property
+    def is_mixed(self):
+        """True for a mesh with hexahedra, wedges or pyramids."""
+        return bool(self._desc.cell_types)
+
+    def elements(self):
+        """(cell_types, cell_offsets, cell_nodes, patch_n, face_types,
+        face_offsets, face_nodes, patch_bc) in either layout."""
+        d = self._desc
+        N, P = d.n_cells, d.n_patches
+        pn = np.ctypeslib.as_array(d.patch_n_faces, shape=(P,)).copy() if P else np.zeros(0, np.uint32)
+        pbc = np.ctypeslib.as_array(d.patch_bc, shape=(P,)).copy() if P else np.zeros(0, np.int32)
+        nf_ = int(pn.sum())
+        if not self.is_mixed:
+            _, cells, _, pf, _ = self.arrays()
+            return (np.full(N, A.NF_ELEM_TETRA, np.uint8), np.arange(N + 1, dtype=np.uint32) * 4, cells.ravel(),
+                    pn, np.full(nf_, A.NF_ELEM_TRIANGLE, np.uint8), np.arange(nf_ + 1, dtype=np.uint32) * 3,
+                    pf.ravel(), pbc)
+        ct = np.ctypeslib.as_array(d.cell_types, shape=(N,)).copy()
+        co = np.ctypeslib.as_array(d.cell_offsets, shape=(N + 1,)).copy()
+        cn = np.ctypeslib.as_array(d.cells, shape=(int(co[-1]),)).copy()
+        ft = np.ctypeslib.as_array(d.patch_face_types, shape=(nf_,)).copy() if nf_ else np.zeros(0, np.uint8)
+        fo = np.ctypeslib.as_array(d.patch_face_offsets, shape=(nf_ + 1,)).copy()
+        fn = np.ctypeslib.as_array(d.patch_faces, shape=(int(fo[-1]),)).copy() if nf_ else np.zeros(0, np.uint32)
+        return ct, co, cn, pn, ft, fo, fn, pbc
+
     def __del__(self):
         if getattr(self, "_h", None):
             A.lib().nf_mesh_destroy(self._h)
@@ -168,6 +222,8 @@ class Mesh:
         return self._desc
 
     def arrays(self):
+        if self.is_mixed:
+            raise ValueError("mixed-element mesh: use elements()")
         d = self._desc
         nodes = np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes, 3)).copy()
         cells = np.ctypeslib.as_array(d.cells, shape=(d.n_cells, 4)).copy()
@@ -183,7 +239,9 @@ class Mesh:
     def fv_grid(self):
         """FV_Grid of this mesh: dict of numpy arrays in the reference face order."""
         s = A.FVSizes()
-        _check(A.lib().nf_fv_sizes_of(self._h, C.byref(s)), what="nf_fv_sizes_of")
+        rc = A.lib().nf_fv_sizes_of(self._h, C.byref(s))
+        if rc != A.NF_OK:
+            raise RuntimeError(f"Error creating grid: {A.lib().nf_mesh_last_error().decode()}")
         F, NT, P = s.n_total_faces, s.n_total_cells, s.n_patches
         out = dict(ij=np.zeros((F, 2), np.uint32), S=np.zeros((F, 3)), r_i=np.zeros((F, 3)),
                    r_j=np.zeros((F, 3)), volume=np.zeros(NT), centroid=np.zeros((NT, 3)),

@@ -10,7 +10,9 @@
 //   face graph           src/geometry/FV_Grid.cpp:21-152, 207-227
 //                        src/geometry/GeometryUtilities.cpp:71-105, 484-513
 //   geometry             src/geometry/FV_Grid.cpp:154-205, 237-267
-//                        src/geometry/GeometryUtilities.cpp:289-329
+//                        src/geometry/GeometryUtilities.cpp:242-441
+//   mixed elements       src/geometry/GeometryUtilities.cpp:48-240,
+//                        include/geometry/GeometryUtilities.hpp:163-191
 //   solver data          src/SolverData.cpp:3-58
 //   physics              include/SolverData.hpp:166-262
 //   step / RK3 / Euler   src/Solver.cpp:34-133
@@ -27,6 +29,7 @@
 // the closed-form and invariant tests in tests/test_oracle.py).
 #include <algorithm>
 #include <cfloat>
+#include <initializer_list>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -353,6 +356,228 @@ void build_grid(Index n_nodes, const double *xyz, Index n_cells, const Index *ce
     }
 }
 
+// ---- mixed-element meshes (hexahedra, wedges, pyramids, tetrahedra) ----
+// Element face tables (src/geometry/GeometryUtilities.cpp:71-240), element
+// and face geometry (:242-441), the face map keyed by node count and sorted
+// nodes (include/geometry/GeometryUtilities.hpp:163-191) and FV_Grid's
+// construction (src/geometry/FV_Grid.cpp:21-205), VTK type numbers.
+struct MFaceKey {
+    int n;
+    Index s[4];
+    bool operator<(const MFaceKey &o) const {  // FaceElement::operator<
+        if (n != o.n) return n < o.n;
+        for (int q = 0; q < n; ++q)
+            if (s[q] != o.s[q]) return s[q] < o.s[q];
+        return false;
+    }
+};
+
+int face_k_of_element(int type, const Index *ve, int k, Index *fe) {
+    auto set = [&](std::initializer_list<int> idx) {
+        int q = 0;
+        for (int v : idx) fe[q++] = ve[v];
+        return q;
+    };
+    if (type == 10) {  // tetrahedron
+        static const int T[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+        return set({T[k][0], T[k][1], T[k][2]});
+    }
+    if (type == 12) {  // hexahedron
+        switch (k) {
+            case 0: return set({0, 1, 2, 3});
+            case 1: return set({4, 5, 6, 7});
+            case 2: return set({0, 1, 5, 4});
+            case 3: return set({3, 2, 6, 7});
+            case 4: return set({3, 0, 4, 7});
+            default: return set({1, 2, 6, 5});
+        }
+    }
+    if (type == 14) {  // pyramid
+        switch (k) {
+            case 0: return set({0, 1, 2, 3});
+            case 1: return set({0, 1, 4});
+            case 2: return set({1, 2, 4});
+            case 3: return set({2, 3, 4});
+            default: return set({3, 0, 4});
+        }
+    }
+    switch (k) {  // wedge
+        case 0: return set({0, 1, 2});
+        case 1: return set({4, 3, 5});
+        case 2: return set({1, 0, 3, 4});
+        case 3: return set({0, 2, 5, 3});
+        default: return set({4, 5, 2, 1});
+    }
+}
+int faces_of(int type) { return type == 10 ? 4 : type == 12 ? 6 : 5; }
+int nodes_of(int type) {
+    switch (type) {
+        case 5: return 3;
+        case 9: return 4;
+        case 10: return 4;
+        case 12: return 8;
+        case 13: return 6;
+        case 14: return 5;
+        default: return 0;
+    }
+}
+
+Vec3 triangle_normal(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2) { return 0.5 * cross(n1 - n0, n2 - n0); }
+Vec3 triangle_centroid(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2) { return ((n0 + n1) + n2) / 3; }
+Vec3 quadrilateral_normal(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2, const Vec3 &n3) {
+    Vec3 S = triangle_normal(n0, n1, n2);
+    return S + triangle_normal(n0, n2, n3);
+}
+Vec3 quadrilateral_centroid(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2, const Vec3 &n3) {
+    Vec3 Sa = triangle_normal(n0, n1, n2), Sb = triangle_normal(n0, n2, n3);
+    Vec3 ca = triangle_centroid(n0, n1, n2), cb = triangle_centroid(n0, n2, n3);
+    double a = std::sqrt(squared_norm(Sa)), b = std::sqrt(squared_norm(Sb));
+    double q = a + b;
+    return (a * ca + b * cb) / q;
+}
+void tetrahedron_props(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2, const Vec3 &n3, double &v, Vec3 &c) {
+    v = dot(cross(n1 - n0, n2 - n0), n3 - n0) / 6;
+    c = 1.0 / 4 * (((n0 + n1) + n2) + n3);
+}
+void pyramid_props(const Vec3 &n0, const Vec3 &n1, const Vec3 &n2, const Vec3 &n3, const Vec3 &n4, double &v,
+                   Vec3 &c) {
+    Vec3 cb = quadrilateral_centroid(n0, n1, n2, n3);
+    Vec3 d = cb - n4;
+    Vec3 Sb = quadrilateral_normal(n0, n3, n2, n1);
+    c = 0.75 * cb + 0.25 * n4;
+    v = dot(d, Sb) / 3;
+}
+void element_props(int type, const std::vector<Vec3> &X, const Index *e, double &vol, Vec3 &cen) {
+    if (type == 10) return tetrahedron_props(X[e[0]], X[e[1]], X[e[2]], X[e[3]], vol, cen);
+    if (type == 14) return pyramid_props(X[e[0]], X[e[1]], X[e[2]], X[e[3]], X[e[4]], vol, cen);
+    vol = 0;
+    cen = {{0, 0, 0}};
+    double v;
+    Vec3 c;
+    auto acc = [&]() {
+        vol += v;
+        cen = cen + v * c;
+    };
+    if (type == 12) {
+        const Vec3 &n0 = X[e[0]], &n1 = X[e[1]], &n2 = X[e[2]], &n3 = X[e[3]], &n4 = X[e[4]], &n5 = X[e[5]],
+                   &n6 = X[e[6]], &n7 = X[e[7]];
+        Vec3 g = ((((((n0 + n1) + n2) + n3) + n4) + n5) + n6 + n7) / 8;
+        pyramid_props(n0, n1, n2, n3, g, v, c), acc();
+        pyramid_props(n4, n7, n6, n5, g, v, c), acc();
+        pyramid_props(n4, n0, n3, n7, g, v, c), acc();
+        pyramid_props(n2, n1, n5, n6, g, v, c), acc();
+        pyramid_props(n1, n0, n4, n5, g, v, c), acc();
+        pyramid_props(n3, n2, n6, n7, g, v, c), acc();
+    } else {  // wedge
+        const Vec3 &n0 = X[e[0]], &n1 = X[e[1]], &n2 = X[e[2]], &n3 = X[e[3]], &n4 = X[e[4]], &n5 = X[e[5]];
+        Vec3 g = ((((n0 + n1) + n2) + n3) + n4 + n5) / 6;
+        tetrahedron_props(n0, n2, n1, g, v, c), acc();
+        tetrahedron_props(n3, n4, n5, g, v, c), acc();
+        pyramid_props(n0, n3, n5, n2, g, v, c), acc();
+        pyramid_props(n0, n1, n4, n3, g, v, c), acc();
+        pyramid_props(n4, n1, n2, n5, g, v, c), acc();
+    }
+    cen = cen / vol;
+}
+
+void build_grid_mixed(Index n_nodes, const double *xyz, Index n_cells, const uint8_t *ctype, const Index *coff,
+                      const Index *conn, Index n_patches, const Index *patch_n, const uint8_t *ftype,
+                      const Index *foff, const Index *fconn, const int *patch_bc, Grid &g) {
+    std::vector<Vec3> X(n_nodes);
+    for (Index n = 0; n < n_nodes; ++n) X[n] = {{xyz[3 * n], xyz[3 * n + 1], xyz[3 * n + 2]}};
+    struct Val {
+        Index cell_i;
+        long cell_j;
+        int n;
+        Index nodes[4];
+    };
+    std::map<MFaceKey, Val> faces_to_cells;
+    auto key_of = [](int n, const Index *f) {
+        MFaceKey k{n, {f[0], f[1], f[2], n == 4 ? f[3] : 0}};
+        std::sort(k.s, k.s + n);
+        return k;
+    };
+    for (Index c = 0; c < n_cells; ++c)
+        for (int k = 0; k < faces_of(ctype[c]); ++k) {
+            Index fe[4] = {0, 0, 0, 0};
+            int n = face_k_of_element(ctype[c], conn + coff[c], k, fe);
+            MFaceKey key = key_of(n, fe);
+            auto it = faces_to_cells.find(key);
+            if (it == faces_to_cells.end()) faces_to_cells.emplace(key, Val{c, -1, n, {fe[0], fe[1], fe[2], fe[3]}});
+            else if (it->second.cell_j != -1) throw std::runtime_error("face shared by three cells");
+            else it->second.cell_j = c;
+        }
+    struct Face {
+        Index i, j;
+        int n;
+        Index nodes[4];
+    };
+    std::vector<Face> faces;
+    for (auto &kv : faces_to_cells)
+        if (kv.second.cell_j != -1)
+            faces.push_back({kv.second.cell_i, (Index)kv.second.cell_j, kv.second.n,
+                             {kv.second.nodes[0], kv.second.nodes[1], kv.second.nodes[2], kv.second.nodes[3]}});
+    Index Fint = (Index)faces.size();
+    Index j_ghost = n_cells, e = 0;
+    for (Index p = 0; p < n_patches; ++p) {
+        g.pbc.push_back(patch_bc[p]);
+        g.pfirst.push_back((Index)faces.size());
+        g.pn.push_back(patch_n[p]);
+        for (Index q = 0; q < patch_n[p]; ++q, ++e) {
+            const Index *f = fconn + foff[e];
+            int n = nodes_of(ftype[e]);
+            auto it = faces_to_cells.find(key_of(n, f));
+            if (it == faces_to_cells.end() || it->second.cell_j != -1)
+                throw std::runtime_error("patch face does not match a boundary face");
+            it->second.cell_j = j_ghost;
+            faces.push_back({it->second.cell_i, j_ghost, n, {f[0], f[1], f[2], n == 4 ? f[3] : 0}});
+            ++j_ghost;
+        }
+    }
+    auto sort_block = [&](Index b, Index e2) {
+        std::stable_sort(faces.begin() + b, faces.begin() + e2,
+                         [](const Face &x, const Face &y) { return x.i != y.i ? x.i < y.i : x.j < y.j; });
+    };
+    sort_block(0, Fint);
+    for (Index p = 0; p < n_patches; ++p) sort_block(g.pfirst[p], g.pfirst[p] + g.pn[p]);
+    g.N = n_cells;
+    g.NT = j_ghost;
+    g.Fint = Fint;
+    g.F = (Index)faces.size();
+    g.vol.assign(g.NT, 0.0);
+    g.cen.assign(g.NT, Vec3{});
+    for (Index c = 0; c < n_cells; ++c) element_props(ctype[c], X, conn + coff[c], g.vol[c], g.cen[c]);
+    g.fi.resize(g.F);
+    g.fj.resize(g.F);
+    g.S.resize(g.F);
+    g.ri.resize(g.F);
+    g.rj.resize(g.F);
+    for (Index f = 0; f < g.F; ++f) {
+        Index i = faces[f].i, j = faces[f].j;
+        g.fi[f] = i;
+        g.fj[f] = j;
+        const Index *fn = faces[f].nodes;
+        Vec3 S, fc;
+        if (faces[f].n == 3) {
+            S = triangle_normal(X[fn[0]], X[fn[1]], X[fn[2]]);
+            fc = triangle_centroid(X[fn[0]], X[fn[1]], X[fn[2]]);
+        } else {
+            S = quadrilateral_normal(X[fn[0]], X[fn[1]], X[fn[2]], X[fn[3]]);
+            fc = quadrilateral_centroid(X[fn[0]], X[fn[1]], X[fn[2]], X[fn[3]]);
+        }
+        if (j >= n_cells) {
+            g.vol[j] = 0.0;
+            g.cen[j] = 2 * fc - g.cen[i];
+        }
+        double nd = dot(S, g.cen[j] - g.cen[i]);
+        if (nd == 0) throw std::runtime_error("degenerate face orientation");
+        if (nd < 0) S = -1 * S;
+        g.S[f] = S;
+        g.ri[f] = fc - g.cen[i];
+        g.rj[f] = fc - g.cen[j];
+    }
+}
+
 struct Params {
     int time_scheme, spatial_order, limiter;
     double CFL;
@@ -609,6 +834,30 @@ void *or_create(uint32_t n_nodes, const double *xyz, uint32_t n_cells, const uin
     try {
         Grid g;
         build_grid(n_nodes, xyz, n_cells, cells, n_patches, patch_n, patch_tri, patch_bc, g);
+        Params prm{p->time_scheme, p->spatial_order, p->limiter, p->CFL, {0, 0, 0, 0, 0},
+                   p->check_physical_validity, p->rk3_zero_old_first_step, p->inv_flux_scheme};
+        std::memcpy(prm.V_inf, p->V_inf, sizeof prm.V_inf);
+        return new Solver(std::move(g), prm);
+    } catch (const std::exception &e) {
+        if (errbuf && errlen > 0) {
+            std::strncpy(errbuf, e.what(), errlen - 1);
+            errbuf[errlen - 1] = 0;
+        }
+        return nullptr;
+    }
+}
+
+// mixed-element mesh: cell i of VTK type ctype[i] with nodes
+// conn[coff[i] .. coff[i+1]), patch face e of type ftype[e] (5 or 9) with
+// nodes fconn[foff[e] .. foff[e+1])
+void *or_create_mixed(uint32_t n_nodes, const double *xyz, uint32_t n_cells, const uint8_t *ctype,
+                      const uint32_t *coff, const uint32_t *conn, uint32_t n_patches, const uint32_t *patch_n,
+                      const uint8_t *ftype, const uint32_t *foff, const uint32_t *fconn, const int32_t *patch_bc,
+                      const or_params *p, char *errbuf, int errlen) {
+    try {
+        Grid g;
+        build_grid_mixed(n_nodes, xyz, n_cells, ctype, coff, conn, n_patches, patch_n, ftype, foff, fconn, patch_bc,
+                         g);
         Params prm{p->time_scheme, p->spatial_order, p->limiter, p->CFL, {0, 0, 0, 0, 0},
                    p->check_physical_validity, p->rk3_zero_old_first_step, p->inv_flux_scheme};
         std::memcpy(prm.V_inf, p->V_inf, sizeof prm.V_inf);

@@ -28,6 +28,10 @@ def lib():
         L.or_create.restype = C.c_void_p
         L.or_create.argtypes = [C.c_uint32, f64p, C.c_uint32, u32p, C.c_uint32, u32p, u32p, i32p,
                                 C.POINTER(OrParams), C.c_char_p, C.c_int]
+        u8p = C.POINTER(C.c_uint8)
+        L.or_create_mixed.restype = C.c_void_p
+        L.or_create_mixed.argtypes = [C.c_uint32, f64p, C.c_uint32, u8p, u32p, u32p, C.c_uint32, u32p, u8p, u32p,
+                                      u32p, i32p, C.POINTER(OrParams), C.c_char_p, C.c_int]
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_sizes.argtypes = [C.c_void_p, u32p]
         L.or_export_grid.argtypes = [C.c_void_p, u32p, f64p, f64p, f64p, f64p, f64p, u32p, u32p, f64p]
@@ -53,7 +57,10 @@ def _p(a, t):
 class Oracle:
     """Reference-restating CPU solver over the same primal mesh arrays."""
 
-    def __init__(self, nodes, cells, patch_n, patch_faces, patch_bc, config):
+    def __init__(self, nodes, cells, patch_n, patch_faces, patch_bc, config, mixed=None):
+        """cells [N][4] and patch_faces [m][3]; or, with mixed = (cell_types,
+        cell_offsets, face_types, face_offsets), the flat node lists of
+        mixed elements."""
         self._keep = [np.ascontiguousarray(nodes, np.float64), np.ascontiguousarray(cells, np.uint32),
                       np.ascontiguousarray(patch_n, np.uint32), np.ascontiguousarray(patch_faces, np.uint32),
                       np.ascontiguousarray(patch_bc, np.int32)]
@@ -69,8 +76,17 @@ class Oracle:
         p.rk3_zero_old_first_step = int(config.rk3_zero_old_first_step)
         p.inv_flux_scheme = {"Rusanov": 0, "HLLC": 1}[config.inviscid_flux_scheme]
         err = C.create_string_buffer(256)
-        self._h = lib().or_create(len(n), _p(n, C.c_double), len(c), _p(c, C.c_uint32), len(pn),
-                                  _p(pn, C.c_uint32), _p(pf, C.c_uint32), _p(pb, C.c_int32), C.byref(p), err, 256)
+        if mixed is None:
+            self._h = lib().or_create(len(n), _p(n, C.c_double), len(c), _p(c, C.c_uint32), len(pn),
+                                      _p(pn, C.c_uint32), _p(pf, C.c_uint32), _p(pb, C.c_int32), C.byref(p), err, 256)
+        else:
+            ct, co, ft, fo = [np.ascontiguousarray(a, t) for a, t in
+                              zip(mixed, (np.uint8, np.uint32, np.uint8, np.uint32))]
+            self._keep += [ct, co, ft, fo]
+            self._h = lib().or_create_mixed(len(n), _p(n, C.c_double), len(ct), _p(ct, C.c_uint8), _p(co, C.c_uint32),
+                                            _p(c, C.c_uint32), len(pn), _p(pn, C.c_uint32), _p(ft, C.c_uint8),
+                                            _p(fo, C.c_uint32), _p(pf, C.c_uint32), _p(pb, C.c_int32), C.byref(p),
+                                            err, 256)
         if not self._h:
             raise RuntimeError(err.value.decode())
         s = np.zeros(5, np.uint32)
@@ -79,6 +95,11 @@ class Oracle:
 
     @classmethod
     def from_mesh(cls, mesh, config):
+        if getattr(mesh, "is_mixed", False):
+            ct, co, cn, pn, ft, fo, fn, pbc = mesh.elements()
+            d = mesh.desc
+            nodes = np.ctypeslib.as_array(d.nodes, shape=(d.n_nodes, 3)).copy()
+            return cls(nodes, cn, pn, fn, pbc, config, mixed=(ct, co, ft, fo))
         nodes, cells, pn, pf, pbc = mesh.arrays()
         return cls(nodes, cells, pn, pf, pbc, config)
 

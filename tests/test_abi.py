@@ -33,7 +33,7 @@ def test_binding_covers_header():
 
 def test_struct_layouts():
     assert C.sizeof(A.Params) == 4 * 4 + 8 + 40 + 4 * 4 + 8  # 5th int + tail padding
-    assert C.sizeof(A.MeshDesc) == 8 * 8
+    assert C.sizeof(A.MeshDesc) == 8 * 8 + 4 * 8  # four mixed-element pointers
     assert C.sizeof(A.RunStats) == 8 + 5 * 8 + 8
     assert C.sizeof(A.RunOpts) == 40
     assert C.sizeof(A.TrainOpts) == 40 and C.sizeof(A.TrainStats) == 24

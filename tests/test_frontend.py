@@ -147,8 +147,12 @@ def test_su2_read_conventions_and_errors(tmp_path):
     assert len(rep["negative_volume"]) == 0
     errors = [
         ({"wall": "SlipWall"}, SU2_ONE_TET, "bottom' is not named"),
-        ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("10 0 2 1 3 0", "12 0 1 2 3 0 1 2 3"),
-         "only tetrahedra"),
+        ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("10 0 2 1 3 0", "3 0 2 1 3 0"),
+         r"Illegal su2 volume element identifier \(3\)"),
+        ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("10 0 2 1 3 0", "5 0 2 1 0"),
+         "Face element of type 5"),
+        ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("5 0 1 2\n", "10 0 1 2 3\n"),
+         "Volume element of type 10"),
         ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("NDIME=3", "NDIME= 2"), "NDIME=3"),
         ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("5 0 1 2\n", "5 0 1 9\n"), "out of range"),
         ({"wall": "SlipWall", "bottom": "FarField"}, SU2_ONE_TET.replace("NPOIN", "NPOINTS"), "NPOIN="),
