@@ -118,6 +118,59 @@ def cpu_baseline(sample_n, steps):
                       f"{len(cpus)} available, {model}, {t:.1f} s"}
 
 
+def poly_bytes_model(N, F, G):
+    """Compulsory HBM bytes of one TVD-RK3 step of the polyhedral path
+    (second order, Barth): per stage prim (U in, V out: 80 N), cell ghosts
+    (V, S, i, j, type in, V out: 113 G), sweep A (face list 24 N, volume 8 N,
+    V 40 N, gradient and limiter out 160 N; per face i, j, S, r_i, r_j 80 B
+    and the two face values out 80 B) and sweep B (face list 24 N, U, U_old,
+    volume in 88 N, U and R out 80 N; per face i, j, S and the two face
+    values 112 B)."""
+    stage = 80 * N + 113 * G + (232 * N + 160 * F) + (192 * N + 112 * F)
+    return 3 * stage
+
+
+def bench_hex(args, nf, pulse_config, pulse_state):
+    """Plain FP64 TVD-RK3 steps on an n^3 hexahedral box (mixed-element
+    path): Driver loop of calc_timestep + step, wall clock over K steps."""
+    n = args.n if args.n != 128 else 96
+    t0 = time.perf_counter()
+    mesh = nf.Mesh.hex_box(n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    cfg = pulse_config(mach=0.5)
+    solver = nf.EulerSolver(cfg, mesh)
+    N, F, Fint = solver.N, solver.sizes.n_total_faces, solver.sizes.n_interior_faces
+    print(f"[bench] hex mesh {n}^3 = {N} cells, {F} faces, setup {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    solver.set_state(U0)
+    for _ in range(args.warmup):
+        solver.calc_timestep(cfg)
+        solver.step(cfg)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.calc_timestep(cfg)
+        solver.step(cfg)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    step_bytes = poly_bytes_model(N, F, F - Fint)
+    out = {
+        "metric": "cell-stage-updates/s",
+        "value": N * 3 * args.steps / wall,
+        "unit": "cell-stage-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (generated jittered hexahedral mesh, Gaussian pulse)",
+        "config": {"workload": f"plain FP64 TVD-RK3 step, mixed-element path, unit cube {n}^3 hexahedra "
+                               f"({N} cells), jitter 0.15, Mach 0.5 + pulse, farfield, CFL 0.5",
+                   "cells": N, "faces": F, "timing": "wall clock, Driver loop (calc_timestep + step)"},
+        "step_roofline_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_algorithmic_bytes": step_bytes,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +191,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="eager launches in the timed region (profiling)")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="pick the sweep launch configuration with the autotuner first")
+    ap.add_argument("--mesh", default="tet", choices=["tet", "hex"],
+                    help="hex: an n^3 hexahedral mesh through the mixed-element (polyhedral) path")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -161,6 +216,9 @@ def main():
 
     import netflux_amd as nf
     from tests.cases import pulse_config, pulse_state
+
+    if args.mesh == "hex":
+        return bench_hex(args, nf, pulse_config, pulse_state)
 
     t0 = time.perf_counter()
     mesh = nf.Mesh.box(args.n, jitter=0.15, seed=SEED, patches=("FarField",) * 6)

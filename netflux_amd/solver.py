@@ -168,15 +168,27 @@ class Mesh:
         gives the tetrahedral layout."""
         ctype_of = {4: A.NF_ELEM_TETRA, 5: A.NF_ELEM_PYRAMID, 6: A.NF_ELEM_WEDGE, 8: A.NF_ELEM_HEXA}
         ftype_of = {3: A.NF_ELEM_TRIANGLE, 4: A.NF_ELEM_QUAD}
+
+        def flat(rows, type_of):
+            """(types, offsets, nodes) of a list of node lists or a 2-D array"""
+            if isinstance(rows, np.ndarray) and rows.ndim == 2:
+                n, k = rows.shape
+                return (np.full(n, type_of[k], np.uint8), (np.arange(n + 1, dtype=np.int64) * k).astype(np.uint32),
+                        np.ascontiguousarray(rows, dtype=np.uint32).ravel())
+            if len(rows) == 0:
+                return np.zeros(0, np.uint8), np.zeros(1, np.uint32), np.zeros(0, np.uint32)
+            return (np.array([type_of[len(r)] for r in rows], dtype=np.uint8),
+                    np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint32),
+                    np.concatenate([np.asarray(r, dtype=np.uint32) for r in rows]))
+
         nodes = np.ascontiguousarray(nodes, dtype=np.float64)
-        ct = np.array([ctype_of[len(c)] for c in cells], dtype=np.uint8)
-        co = np.concatenate([[0], np.cumsum([len(c) for c in cells])]).astype(np.uint32)
-        cn = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.uint32) for c in cells]), dtype=np.uint32)
-        faces = [f for p in patch_faces for f in p]
-        pn = np.array([len(p) for p in patch_faces], dtype=np.uint32)
-        ft = np.array([ftype_of[len(f)] for f in faces], dtype=np.uint8)
-        fo = np.concatenate([[0], np.cumsum([len(f) for f in faces])]).astype(np.uint32)
-        fn = np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=np.uint32) for f in faces]), dtype=np.uint32)
+        ct, co, cn = flat(cells, ctype_of)
+        parts = [flat(p, ftype_of) for p in patch_faces]
+        pn = np.array([len(t) for t, _, _ in parts], dtype=np.uint32)
+        ft = np.concatenate([t for t, _, _ in parts]).astype(np.uint8)
+        fn = np.ascontiguousarray(np.concatenate([v for _, _, v in parts]), dtype=np.uint32)
+        fo = np.concatenate([[0], np.cumsum(np.concatenate([np.diff(o.astype(np.int64)) for _, o, _ in parts]))])
+        fo = fo.astype(np.uint32)
         pbc = np.array([BC_NAMES[b] if isinstance(b, str) else int(b) for b in patch_bc], dtype=np.int32)
         d = A.MeshDesc(len(nodes), _f64(nodes), len(cells), _u32(cn), len(pn), _u32(pn), _u32(fn), _i32(pbc),
                        ct.ctypes.data_as(A.u8p), _u32(co), ft.ctypes.data_as(A.u8p), _u32(fo))
@@ -185,6 +197,42 @@ class Mesh:
         if rc != A.NF_OK:
             raise RuntimeError(f"nf_mesh_from_desc failed ({rc}): {A.lib().nf_mesh_last_error().decode()}")
         return cls(h)
+
+    @classmethod
+    def hex_box(cls, n, lengths=(1.0, 1.0, 1.0), jitter=0.0, seed=20260101, patches=("SlipWall",) * 6):
+        """nx x ny x nz hexahedra (VTK order) on a box, interior nodes moved
+        uniformly by up to jitter x spacing; quadrilateral patches x-, x+,
+        y-, y+, z-, z+."""
+        nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+        h = np.array(lengths, float) / np.array([nx, ny, nz], float)
+        I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+        X = np.stack([I * h[0], J * h[1], K * h[2]], -1).reshape(-1, 3)
+        if jitter > 0:
+            inner = ((I > 0) & (I < nx) & (J > 0) & (J < ny) & (K > 0) & (K < nz)).ravel()
+            rng = np.random.default_rng(seed)
+            X[inner] += (rng.random((int(inner.sum()), 3)) - 0.5) * 2 * jitter * h
+
+        def idx(i, j, k):
+            return ((i * (ny + 1) + j) * (nz + 1) + k).astype(np.int64)
+
+        i, j, k = [a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")]
+        cells = np.stack([idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k), idx(i, j + 1, k),
+                          idx(i, j, k + 1), idx(i + 1, j, k + 1), idx(i + 1, j + 1, k + 1), idx(i, j + 1, k + 1)], 1)
+
+        def side(axis, end):
+            dims = [nx, ny, nz]
+            a, b = [d for d in range(3) if d != axis]
+            u, v = [w.ravel() for w in np.meshgrid(np.arange(dims[a]), np.arange(dims[b]), indexing="ij")]
+            c = np.full_like(u, 0 if end == 0 else dims[axis])
+
+            def at(du, dv):
+                q = [None] * 3
+                q[axis], q[a], q[b] = c, u + du, v + dv
+                return idx(*q)
+            return np.stack([at(0, 0), at(1, 0), at(1, 1), at(0, 1)], 1)
+
+        faces = [side(ax, e) for ax in range(3) for e in (0, 1)]
+        return cls.from_elements(X, cells, faces, list(patches))
 
     @property
     def is_mixed(self):
