@@ -120,13 +120,14 @@ def cpu_baseline(sample_n, steps):
 
 def poly_bytes_model(N, F, G):
     """Compulsory HBM bytes of one TVD-RK3 step of the polyhedral path
-    (second order, Barth): per stage prim (U in, V out: 80 N), cell ghosts
-    (V, S, i, j, type in, V out: 113 G), sweep A (face list 24 N, volume 8 N,
-    V 40 N, gradient and limiter out 160 N; per face i, j, S, r_i, r_j 80 B
-    and the two face values out 80 B) and sweep B (face list 24 N, U, U_old,
-    volume in 88 N, U and R out 80 N; per face i, j, S and the two face
-    values 112 B)."""
-    stage = 80 * N + 113 * G + (232 * N + 160 * F) + (192 * N + 112 * F)
+    (second order, Barth) in its per-cell slot layout, with
+    slots = 2 (F - G) + G cell-face pairs: per stage prim (U in, V out: 80 N),
+    cell ghosts (i, j, S, type, V in, V out: 113 G), sweep A (slot words
+    24 N, V and volume 48 N; per slot S and r 48 B in, face value 40 B out)
+    and sweep B (slot words and neighbour slots 30 N, U, U_old, volume 88 N,
+    U out 40 N; per slot S 24 B and its face value 40 B)."""
+    slots = 2 * (F - G) + G
+    stage = 80 * N + 113 * G + (72 * N + 88 * slots) + (158 * N + 64 * slots)
     return 3 * stage
 
 

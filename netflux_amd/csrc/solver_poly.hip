@@ -1,17 +1,19 @@
-// Mixed-element FP64 step on the GPU (poly.hpp).  Layouts, all in the
-// reference's cell and face numbering:
-//   U, U_old, R      [5][N]     conservative state, old state, flux balance
-//   V                [5][NT]    primvars of the cells, then of the ghosts
-//   grad             [15][N]    Green-Gauss gradient, index k + 5 d
-//   psi              [5][N]     Barth limiter
-//   VL / VR          [5][F] / [5][Fint]  face values reconstructed from
-//                               the cell i / the cell j side of each face
-//   cf               [6][N]     faces of each cell in increasing face index,
-//                               bit 31 = the cell is the face's j side
+// Mixed-element FP64 step on the GPU (poly.hpp).  Per-cell slot layout:
+// slot s of cell c is its s-th face in increasing reference face index, and
+// every per-slot array is [slot][...][N] so a wave's 64 cells read each
+// slot field with one coalesced load.
+//   w      [6][N]     NONE, or bit 31 = the cell is the face's j side,
+//                     bit 30 = boundary, bits 0-29 = neighbour cell or ghost id
+//   ns     [6][N]     interior: the neighbour's slot of the same face;
+//                     boundary: the boundary type
+//   geo    [6][6][N]  S of the face (as stored, i -> j), then r of this side
+//   U, U_old, R  [5][N];  V [5][NT] (cells, then ghosts);  FV [6][5][N]
+//   reconstructed face values of each slot;  grad [15][N], psi [5][N]
 // Kernels per stage: prim (U -> V), cell ghosts, sweep A (gradient, min/max,
 // limiter, face values; second order only), sweep B (flux balance and the
 // stage update).  Each follows the CPU solver's loops (src/Solver.cpp:54-230,
-// include/Numerics.hpp:86-292) with the same operations in the same order.
+// include/Numerics.hpp:86-292) with the same operations in the same order;
+// gradient, limiter and R are stored only when they are to be exported.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,14 +36,19 @@ constexpr uint32_t NONE = 0xFFFFFFFFu, SIDE_J = 0x80000000u;
 using V3 = Vec3T<double>;
 
 struct PolyDev {
-    uint32_t N, NT, Fint, F;
-    const uint32_t *cf, *fi, *fj;
-    const double *S, *ri, *rj, *vol, *dS;
+    uint32_t N, NT, G;
+    const uint32_t *w;
+    const uint8_t *ns;
+    const double *geo, *vol, *dS;
+    // boundary faces, for the cell ghosts: cell i, ghost j, S, type
+    const uint32_t *bi, *bj;
+    const double *bS;
     const uint8_t *bbc;
 };
+constexpr uint32_t BND = 0x40000000u, IDX = 0x3FFFFFFFu;
 
-__device__ __forceinline__ V3 ld3(const double *a, uint32_t stride, uint32_t i) {
-    return {a[i], a[stride + i], a[2 * (size_t)stride + i]};
+__device__ __forceinline__ double geo(const PolyDev &m, int s, int q, uint32_t c) {
+    return m.geo[((size_t)s * 6 + q) * m.N + c];
 }
 
 __global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, uint32_t N, uint32_t NT) {
@@ -56,28 +63,29 @@ __global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, ui
 // set_constant_ghost_values (src/Solver.cpp:300-330)
 __global__ __launch_bounds__(PB) void k_poly_ghost(PolyDev m, double *V, FarField<double> fs) {
     const uint32_t b = blockIdx.x * PB + threadIdx.x;
-    if (b >= m.F - m.Fint) return;
-    const uint32_t f = m.Fint + b, i = m.fi[f], j = m.fj[f];
+    if (b >= m.G) return;
+    const uint32_t i = m.bi[b], j = m.bj[b];
     double vd[5], vg[5];
     for (int k = 0; k < 5; ++k) vd[k] = V[k * (size_t)m.NT + i];
-    ghost_value<double>((int)m.bbc[b], vd, ld3(m.S, m.F, f), fs, vg);
+    const V3 S = {m.bS[b], m.bS[m.G + b], m.bS[2 * (size_t)m.G + b]};
+    ghost_value<double>((int)m.bbc[b], vd, S, fs, vg);
     for (int k = 0; k < 5; ++k) V[k * (size_t)m.NT + j] = vg[k];
 }
 
 // Green-Gauss (Numerics.hpp:86-128), max/min (:253-292), Barth (:148-250)
 // and the reconstructed face values (Solver.hpp:81-112) of one cell
-__global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, int limiter, double *grad,
-                                                    double *psi, double *VL, double *VR) {
+__global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, int limiter, double *FV,
+                                                    double *grad, double *psi) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     if (c >= m.N) return;
-    const size_t NT = m.NT;
+    const size_t NT = m.NT, N = m.N;
     double vc[5];
     for (int k = 0; k < 5; ++k) vc[k] = V[k * NT + c];
     const double vol = m.vol[c];
     uint32_t w[6];
     int nfc = 0;
     for (int s = 0; s < 6; ++s) {
-        w[s] = m.cf[s * (size_t)m.N + c];
+        w[s] = m.w[s * N + c];
         if (w[s] != NONE) nfc = s + 1;
     }
     double g[15];
@@ -85,14 +93,14 @@ __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, 
     double vmax[5], vmin[5];
     for (int k = 0; k < 5; ++k) vmax[k] = -DBL_MAX, vmin[k] = DBL_MAX;
     for (int s = 0; s < nfc; ++s) {
-        const uint32_t f = w[s] & ~SIDE_J;
+        const uint32_t n = w[s] & IDX;
         const bool sj = (w[s] & SIDE_J) != 0;
-        const uint32_t i = m.fi[f], j = m.fj[f];
-        const V3 S = ld3(m.S, m.F, f);
+        const double Sx = geo(m, s, 0, c), Sy = geo(m, s, 1, c), Sz = geo(m, s, 2, c);
         for (int k = 0; k < 5; ++k) {
-            const double vi = V[k * NT + i], vj = V[k * NT + j];
+            const double vn = V[k * NT + n];
+            const double vi = sj ? vn : vc[k], vj = sj ? vc[k] : vn;  // the face's i and j sides
             const double uf = 0.5 * (vi + vj);
-            const double t0 = uf * S.x, t1 = uf * S.y, t2 = uf * S.z;
+            const double t0 = uf * Sx, t1 = uf * Sy, t2 = uf * Sz;
             if (!sj) {
                 g[k] += t0 / vol;
                 g[k + 5] += t1 / vol;
@@ -110,10 +118,9 @@ __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, 
     if (limiter) {
         for (int k = 0; k < 5; ++k) ps[k] = DBL_MAX;
         for (int s = 0; s < nfc; ++s) {
-            const uint32_t f = w[s] & ~SIDE_J;
-            const V3 r = ld3((w[s] & SIDE_J) ? m.rj : m.ri, m.F, f);
+            const double rx = geo(m, s, 3, c), ry = geo(m, s, 4, c), rz = geo(m, s, 5, c);
             for (int k = 0; k < 5; ++k) {
-                double D = (g[k] * r.x + g[k + 5] * r.y) + g[k + 10] * r.z;
+                double D = (g[k] * rx + g[k + 5] * ry) + g[k + 10] * rz;
                 D = (double)isign(D) * (std::fabs(D) + DBL_EPSILON);
                 double fcon;
                 if (D > 0.0) fcon = smin(1.0, (vmax[k] - vc[k]) / D);
@@ -126,25 +133,23 @@ __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, 
         for (int k = 0; k < 5; ++k) ps[k] = 1.0;  // SolverData.cpp:17
     }
     for (int s = 0; s < nfc; ++s) {
-        const uint32_t f = w[s] & ~SIDE_J;
-        const bool sj = (w[s] & SIDE_J) != 0;
-        const V3 r = ld3(sj ? m.rj : m.ri, m.F, f);
+        const double rx = geo(m, s, 3, c), ry = geo(m, s, 4, c), rz = geo(m, s, 5, c);
         for (int k = 0; k < 5; ++k) {
-            const double D = (g[k] * r.x + g[k + 5] * r.y) + g[k + 10] * r.z;
-            const double vf = vc[k] + ps[k] * D;
-            if (sj) VR[k * (size_t)m.Fint + f] = vf;
-            else VL[k * (size_t)m.F + f] = vf;
+            const double D = (g[k] * rx + g[k + 5] * ry) + g[k + 10] * rz;
+            FV[((size_t)s * 5 + k) * N + c] = vc[k] + ps[k] * D;
         }
     }
-    for (int q = 0; q < 15; ++q) grad[q * (size_t)m.N + c] = g[q];
-    for (int k = 0; k < 5; ++k) psi[k * (size_t)m.N + c] = ps[k];
+    if (grad) {
+        for (int q = 0; q < 15; ++q) grad[q * N + c] = g[q];
+        for (int k = 0; k < 5; ++k) psi[k * N + c] = ps[k];
+    }
 }
 
 struct StageArgs {
-    const double *V, *VL, *VR;  // VL == nullptr: first order (face values = cell values)
+    const double *V, *FV;  // FV == nullptr: first order (face values = cell values)
     double *U;
     const double *Uo;
-    double *R;
+    double *R;   // stored when not null
     int *bad;
     int flux;    // 0 Rusanov, 1 HLLC
     int kind;    // 0: U + dt/vol R, 1: RK3 stage 2, 2: RK3 stage 3, -1: no update
@@ -157,26 +162,27 @@ struct StageArgs {
 __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     if (c >= m.N) return;
-    const size_t NT = m.NT;
+    const size_t NT = m.NT, N = m.N;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < 6; ++s) {
-        const uint32_t w = m.cf[s * (size_t)m.N + c];
+        const uint32_t w = m.w[s * N + c];
         if (w == NONE) break;
-        const uint32_t f = w & ~SIDE_J;
-        const V3 S = ld3(m.S, m.F, f);
-        double vl[5], vr[5], ul[5], ur[5], F[5];
-        const uint32_t i = m.fi[f], j = m.fj[f];
-        for (int k = 0; k < 5; ++k) vl[k] = a.VL ? a.VL[k * (size_t)m.F + f] : a.V[k * NT + i];
-        if (f < m.Fint) {
-            for (int k = 0; k < 5; ++k) vr[k] = a.VL ? a.VR[k * (size_t)m.Fint + f] : a.V[k * NT + j];
+        const uint32_t n = w & IDX;
+        const uint32_t q = m.ns[s * N + c];
+        const V3 S = {geo(m, s, 0, c), geo(m, s, 1, c), geo(m, s, 2, c)};
+        double own[5], oth[5], ul[5], ur[5], F[5];
+        for (int k = 0; k < 5; ++k) own[k] = a.FV ? a.FV[((size_t)s * 5 + k) * N + c] : a.V[k * NT + c];
+        if (w & BND) {  // the cell is the face's i side; ghost of its face value
+            ghost_value<double>((int)q, own, S, a.fs, oth);
         } else {
-            ghost_value<double>((int)m.bbc[f - m.Fint], vl, S, a.fs, vr);
+            for (int k = 0; k < 5; ++k) oth[k] = a.FV ? a.FV[((size_t)q * 5 + k) * N + n] : a.V[k * NT + n];
         }
-        prim_to_cons(vl, ul);
-        prim_to_cons(vr, ur);
+        const bool sj = (w & SIDE_J) != 0;
+        prim_to_cons(sj ? oth : own, ul);
+        prim_to_cons(sj ? own : oth, ur);
         if (a.flux == 1) hllc_d(ul, ur, S, F);
         else rusanov<double>(ul, ur, S, F);
-        if (w & SIDE_J) {
+        if (sj) {
             for (int k = 0; k < 5; ++k) R[k] += F[k];
         } else {
             for (int k = 0; k < 5; ++k) R[k] -= F[k];
@@ -184,14 +190,14 @@ __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
     }
     bool ok = true;
     for (int k = 0; k < 5; ++k) {
-        a.R[k * (size_t)m.N + c] = R[k];
+        if (a.R) a.R[k * N + c] = R[k];
         ok = ok && std::isfinite(R[k]);
     }
     if (!ok && a.bad) atomicAdd(a.bad, 1);
     if (a.kind < 0) return;
     const double vol = m.vol[c];
     for (int k = 0; k < 5; ++k) {
-        const size_t q = k * (size_t)m.N + c;
+        const size_t q = k * N + c;
         const double u = a.U[q];
         if (a.kind == 0) a.U[q] = u + a.dt / vol * R[k];
         else if (a.kind == 1) a.U[q] = 3.0 / 4.0 * a.Uo[q] + 1.0 / 4.0 * u + 1.0 / 4.0 * a.dt / vol * R[k];
@@ -250,7 +256,7 @@ struct PolyPath {
     FarField<double> fs{};
     PolyDev m{};
     nf_fv_sizes sizes{};
-    double *U = nullptr, *Uo = nullptr, *V = nullptr, *Ve = nullptr, *VL = nullptr, *VR = nullptr;
+    double *U = nullptr, *Uo = nullptr, *V = nullptr, *Ve = nullptr, *FV = nullptr;
     double *grad = nullptr, *psi = nullptr, *R = nullptr, *part = nullptr;
     int *bad = nullptr;
     bool aux_valid = false;
@@ -304,23 +310,22 @@ int get_soa(PolyPath *P, const double *src, int cols, double *aos, std::string &
 
 // evaluate_flux_balance of U into R (and the stage update when kind >= 0):
 // primvars into V, ghosts, sweep A for second order, sweep B
-int flux_balance(PolyPath *P, double *V, int kind, double dt, std::string &err) {
+int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, std::string &err) {
     const PolyDev &m = P->m;
-    const uint32_t G = m.F - m.Fint;
     const bool check = P->prm.check_physical_validity != 0;
     hipLaunchKernelGGL(k_poly_prim, grid(m.N), dim3(PB), 0, P->st, (const double *)P->U, V, m.N, m.NT);
-    if (G) hipLaunchKernelGGL(k_poly_ghost, grid(G), dim3(PB), 0, P->st, m, V, P->fs);
+    if (m.G) hipLaunchKernelGGL(k_poly_ghost, grid(m.G), dim3(PB), 0, P->st, m, V, P->fs);
     const bool second = P->prm.spatial_order == NF_SECOND_ORDER;
     if (second)
         hipLaunchKernelGGL(k_poly_sweepA, grid(m.N), dim3(PB), 0, P->st, m, (const double *)V,
-                           P->prm.limiter != NF_LIMITER_NONE ? 1 : 0, P->grad, P->psi, P->VL, P->VR);
+                           P->prm.limiter != NF_LIMITER_NONE ? 1 : 0, P->FV, aux ? P->grad : nullptr,
+                           aux ? P->psi : nullptr);
     StageArgs a;
     a.V = V;
-    a.VL = second ? P->VL : nullptr;
-    a.VR = second ? P->VR : nullptr;
+    a.FV = second ? P->FV : nullptr;
     a.U = P->U;
     a.Uo = P->Uo;
-    a.R = P->R;
+    a.R = (aux || check) ? P->R : nullptr;
     a.bad = check ? P->bad : nullptr;
     a.flux = P->prm.inv_flux_scheme == NF_FLUX_HLLC ? 1 : 0;
     a.kind = check ? -1 : kind;
@@ -367,61 +372,73 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
     P->fs = make_farfield<double>(p.primvars_inf);
     const uint32_t N = g.N, NT = g.NT, Fint = g.Fint, F = g.F;
     P->sizes = {N, NT, Fint, F, (uint32_t)g.pfirst.size()};
-    // faces of each cell in increasing face index
-    std::vector<uint32_t> cf((size_t)6 * N, NONE);
-    std::vector<uint8_t> cnt(N, 0);
-    std::vector<uint32_t> fi(F), fj(F);
+    if (NT - 1 > IDX) {
+        err = "mesh too large for the mixed-element path";
+        return NF_E_MESH;
+    }
+    // slots: the faces of each cell in increasing face index
+    std::vector<uint32_t> w((size_t)6 * N, NONE);
+    std::vector<uint8_t> ns((size_t)6 * N, 0), cnt(N, 0);
+    std::vector<double> geo((size_t)36 * N, 0.0);
+    const uint32_t G = F - Fint;
+    std::vector<uint32_t> bi(G), bj(G);
+    std::vector<double> bS((size_t)3 * G);
+    std::vector<uint8_t> bbc(G);
+    for (size_t q = 0; q < g.pfirst.size(); ++q)
+        for (uint32_t f = g.pfirst[q]; f < g.pfirst[q] + g.pn[q]; ++f) bbc[f - Fint] = (uint8_t)g.pbc[q];
+    auto put_geo = [&](uint32_t c, int s, uint32_t f, const std::vector<double> &r) {
+        for (int d = 0; d < 3; ++d) {
+            geo[((size_t)s * 6 + d) * N + c] = g.S[3 * (size_t)f + d];
+            geo[((size_t)s * 6 + 3 + d) * N + c] = r[3 * (size_t)f + d];
+        }
+    };
     for (uint32_t f = 0; f < F; ++f) {
         const uint32_t i = g.ij[2 * (size_t)f], j = g.ij[2 * (size_t)f + 1];
-        fi[f] = i;
-        fj[f] = j;
-        if (cnt[i] >= 6) {
+        if (cnt[i] >= 6 || (j < N && cnt[j] >= 6)) {
             err = "cell with more than six faces";
             return NF_E_MESH;
         }
-        cf[cnt[i]++ * (size_t)N + i] = f;
+        const int si = cnt[i]++;
         if (j < N) {
-            if (cnt[j] >= 6) {
-                err = "cell with more than six faces";
-                return NF_E_MESH;
-            }
-            cf[cnt[j]++ * (size_t)N + j] = f | SIDE_J;
+            const int sj = cnt[j]++;
+            w[(size_t)si * N + i] = j;
+            ns[(size_t)si * N + i] = (uint8_t)sj;
+            w[(size_t)sj * N + j] = i | SIDE_J;
+            ns[(size_t)sj * N + j] = (uint8_t)si;
+            put_geo(j, sj, f, g.rj);
+        } else {
+            const uint32_t b = f - Fint;
+            w[(size_t)si * N + i] = j | BND;
+            ns[(size_t)si * N + i] = bbc[b];
+            bi[b] = i;
+            bj[b] = j;
+            for (int d = 0; d < 3; ++d) bS[(size_t)d * G + b] = g.S[3 * (size_t)f + d];
         }
+        put_geo(i, si, f, g.ri);
     }
-    auto soa3 = [&](const std::vector<double> &a, uint32_t n) {
-        std::vector<double> o((size_t)3 * n);
-        for (uint32_t q = 0; q < n; ++q)
-            for (int d = 0; d < 3; ++d) o[d * (size_t)n + q] = a[3 * (size_t)q + d];
-        return o;
-    };
-    std::vector<uint8_t> bbc(F - Fint);
-    for (size_t q = 0; q < g.pfirst.size(); ++q)
-        for (uint32_t f = g.pfirst[q]; f < g.pfirst[q] + g.pn[q]; ++f) bbc[f - Fint] = (uint8_t)g.pbc[q];
     std::vector<double> vol(g.vol.begin(), g.vol.begin() + N);
     std::vector<double> dS((size_t)3 * N);
     for (uint32_t c = 0; c < N; ++c)
         for (int d = 0; d < 3; ++d) dS[d * (size_t)N + c] = g.dS[3 * (size_t)c + d];
-    uint32_t *dcf, *dfi, *dfj;
-    double *dSv, *dri, *drj, *dvol, *ddS;
-    uint8_t *dbbc;
+    uint32_t *dw, *dbi, *dbj;
+    uint8_t *dns, *dbbc;
+    double *dgeo, *dvol, *ddS, *dbS;
     int rc;
-    if ((rc = alloc(P, &dcf, cf.size(), err)) || (rc = alloc(P, &dfi, F, err)) || (rc = alloc(P, &dfj, F, err)) ||
-        (rc = alloc(P, &dSv, 3 * (size_t)F, err)) || (rc = alloc(P, &dri, 3 * (size_t)F, err)) ||
-        (rc = alloc(P, &drj, 3 * (size_t)F, err)) || (rc = alloc(P, &dvol, N, err)) ||
-        (rc = alloc(P, &ddS, 3 * (size_t)N, err)) || (rc = alloc(P, &dbbc, bbc.size(), err)))
+    if ((rc = alloc(P, &dw, w.size(), err)) || (rc = alloc(P, &dns, ns.size(), err)) ||
+        (rc = alloc(P, &dgeo, geo.size(), err)) || (rc = alloc(P, &dvol, N, err)) ||
+        (rc = alloc(P, &ddS, 3 * (size_t)N, err)) || (rc = alloc(P, &dbi, G, err)) || (rc = alloc(P, &dbj, G, err)) ||
+        (rc = alloc(P, &dbS, 3 * (size_t)G, err)) || (rc = alloc(P, &dbbc, G, err)))
         return rc;
-    if ((rc = upload(dcf, cf, err)) || (rc = upload(dfi, fi, err)) || (rc = upload(dfj, fj, err)) ||
-        (rc = upload(dSv, soa3(g.S, F), err)) || (rc = upload(dri, soa3(g.ri, F), err)) ||
-        (rc = upload(drj, soa3(g.rj, F), err)) || (rc = upload(dvol, vol, err)) || (rc = upload(ddS, dS, err)) ||
-        (rc = upload(dbbc, bbc, err)))
+    if ((rc = upload(dw, w, err)) || (rc = upload(dns, ns, err)) || (rc = upload(dgeo, geo, err)) ||
+        (rc = upload(dvol, vol, err)) || (rc = upload(ddS, dS, err)) || (rc = upload(dbi, bi, err)) ||
+        (rc = upload(dbj, bj, err)) || (rc = upload(dbS, bS, err)) || (rc = upload(dbbc, bbc, err)))
         return rc;
-    P->m = {N, NT, Fint, F, dcf, dfi, dfj, dSv, dri, drj, dvol, ddS, dbbc};
+    P->m = {N, NT, G, dw, dns, dgeo, dvol, ddS, dbi, dbj, dbS, dbbc};
     if ((rc = alloc(P, &P->U, 5 * (size_t)N, err)) || (rc = alloc(P, &P->Uo, 5 * (size_t)N, err)) ||
         (rc = alloc(P, &P->V, 5 * (size_t)NT, err)) || (rc = alloc(P, &P->Ve, 5 * (size_t)NT, err)) ||
-        (rc = alloc(P, &P->VL, 5 * (size_t)F, err)) || (rc = alloc(P, &P->VR, 5 * (size_t)Fint, err)) ||
-        (rc = alloc(P, &P->grad, 15 * (size_t)N, err)) || (rc = alloc(P, &P->psi, 5 * (size_t)N, err)) ||
-        (rc = alloc(P, &P->R, 5 * (size_t)N, err)) || (rc = alloc(P, &P->part, (N + PB - 1) / PB, err)) ||
-        (rc = alloc(P, &P->bad, 1, err)))
+        (rc = alloc(P, &P->FV, 30 * (size_t)N, err)) || (rc = alloc(P, &P->grad, 15 * (size_t)N, err)) ||
+        (rc = alloc(P, &P->psi, 5 * (size_t)N, err)) || (rc = alloc(P, &P->R, 5 * (size_t)N, err)) ||
+        (rc = alloc(P, &P->part, (N + PB - 1) / PB, err)) || (rc = alloc(P, &P->bad, 1, err)))
         return rc;
     if ((rc = poly_set_freestream(P, err))) return rc;
     *out = hold.release();
@@ -497,10 +514,10 @@ int poly_calc_timestep(PolyPath *P, double *dt_out, std::string &err) {
 // Solver::step (src/Solver.cpp:34-52, 83-133)
 int poly_step(PolyPath *P, double dt, std::string &err) {
     int rc;
-    if ((rc = flux_balance(P, P->V, 0, dt, err))) return rc;
+    if ((rc = flux_balance(P, P->V, 0, dt, false, err))) return rc;
     if (P->prm.time_scheme == NF_TVD_RK3) {
-        if ((rc = flux_balance(P, P->V, 1, dt, err))) return rc;
-        if ((rc = flux_balance(P, P->V, 2, dt, err))) return rc;
+        if ((rc = flux_balance(P, P->V, 1, dt, false, err))) return rc;
+        if ((rc = flux_balance(P, P->V, 2, dt, false, err))) return rc;
     }
     PHIP(hipMemcpyAsync(P->Uo, P->U, 5 * (size_t)P->m.N * 8, hipMemcpyDeviceToDevice, P->st));
     PHIP(hipStreamSynchronize(P->st));
@@ -511,7 +528,7 @@ int poly_step(PolyPath *P, double dt, std::string &err) {
 // the primvars a following calc_timestep reads are left as the last step set them
 int poly_eval_flux_balance(PolyPath *P, std::string &err) {
     int rc;
-    if ((rc = flux_balance(P, P->Ve, -1, 0.0, err))) return rc;
+    if ((rc = flux_balance(P, P->Ve, -1, 0.0, true, err))) return rc;
     PHIP(hipStreamSynchronize(P->st));
     P->aux_valid = true;
     return NF_OK;
