@@ -28,6 +28,8 @@ int poly_calc_timestep(PolyPath *P, double *dt, std::string &err);
 int poly_step(PolyPath *P, double dt, std::string &err);
 int poly_eval_flux_balance(PolyPath *P, std::string &err);
 int poly_get_state(PolyPath *P, double *U, std::string &err);
+int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err);
+int poly_get_residuals(PolyPath *P, double *out, int32_t max, int32_t *n_out);
 int poly_get_field(PolyPath *P, int32_t which, double *out, std::string &err);
 
 }  // namespace nf

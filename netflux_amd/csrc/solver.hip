@@ -975,8 +975,8 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
 }
 
 const char *POLY_UNSUPPORTED =
-    "not available for a mixed-element mesh (supported: set_freestream, set_state, calc_timestep, step, "
-    "eval_flux_balance, get_state, get_field, ctx_sizes)";
+    "not available for a mixed-element mesh (supported: set_freestream, set_state, calc_timestep, step, run, "
+    "get_residuals, eval_flux_balance, get_state, get_field, ctx_sizes)";
 bool single(const nf_ctx *ctx) {
     if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
     return ctx && ctx->n_inst == 1;
@@ -1269,6 +1269,7 @@ int nf_step(nf_ctx *ctx, double dt) {
 }
 
 int nf_run(nf_ctx *ctx, const nf_run_opts *o, nf_run_stats *st) {
+    if (ctx && ctx->poly) return nf::poly_run(ctx->poly, o, st, ctx->err);
     if (!single(ctx)) return NF_E_ARG;
     return run_loop(ctx, o, st);
 }
@@ -1293,6 +1294,7 @@ int nf_inst_residuals(nf_ctx *ctx, int32_t i, double *out, int32_t max, int32_t 
 }
 
 int nf_get_residuals(nf_ctx *ctx, double *out, int32_t max, int32_t *n_out) {
+    if (ctx && ctx->poly) return out ? nf::poly_get_residuals(ctx->poly, out, max, n_out) : NF_E_ARG;
     if (!single(ctx)) return NF_E_ARG;
     return nf_inst_residuals(ctx, 0, out, max, n_out);
 }

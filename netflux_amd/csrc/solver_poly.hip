@@ -149,7 +149,8 @@ struct StageArgs {
     const double *V, *FV;  // FV == nullptr: first order (face values = cell values)
     double *U;
     const double *Uo;
-    double *R;   // stored when not null
+    double *R;     // stored when not null
+    double *Rrho;  // the density flux balance, stored when not null (residual norm)
     int *bad;
     int flux;    // 0 Rusanov, 1 HLLC
     int kind;    // 0: U + dt/vol R, 1: RK3 stage 2, 2: RK3 stage 3, -1: no update
@@ -193,6 +194,7 @@ __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
         if (a.R) a.R[k * N + c] = R[k];
         ok = ok && std::isfinite(R[k]);
     }
+    if (a.Rrho) a.Rrho[c] = R[0];
     if (!ok && a.bad) atomicAdd(a.bad, 1);
     if (a.kind < 0) return;
     const double vol = m.vol[c];
@@ -244,6 +246,19 @@ __global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, doub
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// per-block sums of x^2 (the residual norm), fixed order
+__global__ __launch_bounds__(PB) void k_poly_sumsq(const double *x, uint32_t n, double *part) {
+    __shared__ double sh[PB];
+    const uint32_t c = blockIdx.x * PB + threadIdx.x;
+    sh[threadIdx.x] = c < n ? x[c] * x[c] : 0.0;
+    __syncthreads();
+    for (int h = PB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 __global__ __launch_bounds__(PB) void k_poly_fill(double *p, double v, size_t n) {
     const size_t q = (size_t)blockIdx.x * PB + threadIdx.x;
     if (q < n) p[q] = v;
@@ -258,8 +273,13 @@ struct PolyPath {
     nf_fv_sizes sizes{};
     double *U = nullptr, *Uo = nullptr, *V = nullptr, *Ve = nullptr, *FV = nullptr;
     double *grad = nullptr, *psi = nullptr, *R = nullptr, *part = nullptr;
+    double *Rrho = nullptr;
     int *bad = nullptr;
     bool aux_valid = false;
+    // run state since the last set_state / set_freestream
+    std::vector<double> hist;  // residual norm of each step's last stage
+    double time = 0.0;
+    bool converged = false;
     std::vector<void *> owned;
     ~PolyPath() {
         if (st) (void)hipStreamSynchronize(st);
@@ -310,7 +330,7 @@ int get_soa(PolyPath *P, const double *src, int cols, double *aos, std::string &
 
 // evaluate_flux_balance of U into R (and the stage update when kind >= 0):
 // primvars into V, ghosts, sweep A for second order, sweep B
-int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, std::string &err) {
+int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, std::string &err) {
     const PolyDev &m = P->m;
     const bool check = P->prm.check_physical_validity != 0;
     hipLaunchKernelGGL(k_poly_prim, grid(m.N), dim3(PB), 0, P->st, (const double *)P->U, V, m.N, m.NT);
@@ -326,6 +346,7 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, std::str
     a.U = P->U;
     a.Uo = P->Uo;
     a.R = (aux || check) ? P->R : nullptr;
+    a.Rrho = res ? P->Rrho : nullptr;
     a.bad = check ? P->bad : nullptr;
     a.flux = P->prm.inv_flux_scheme == NF_FLUX_HLLC ? 1 : 0;
     a.kind = check ? -1 : kind;
@@ -438,7 +459,8 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
         (rc = alloc(P, &P->V, 5 * (size_t)NT, err)) || (rc = alloc(P, &P->Ve, 5 * (size_t)NT, err)) ||
         (rc = alloc(P, &P->FV, 30 * (size_t)N, err)) || (rc = alloc(P, &P->grad, 15 * (size_t)N, err)) ||
         (rc = alloc(P, &P->psi, 5 * (size_t)N, err)) || (rc = alloc(P, &P->R, 5 * (size_t)N, err)) ||
-        (rc = alloc(P, &P->part, (N + PB - 1) / PB, err)) || (rc = alloc(P, &P->bad, 1, err)))
+        (rc = alloc(P, &P->part, (N + PB - 1) / PB, err)) || (rc = alloc(P, &P->bad, 1, err)) ||
+        (rc = alloc(P, &P->Rrho, N, err)))
         return rc;
     if ((rc = poly_set_freestream(P, err))) return rc;
     *out = hold.release();
@@ -450,8 +472,15 @@ void poly_destroy(PolyPath *P) { delete P; }
 nf_fv_sizes poly_sizes(const PolyPath *P) { return P->sizes; }
 
 // set_freestream_values (src/SolverData.cpp:50-58)
+static void reset_run(PolyPath *P) {
+    P->hist.clear();
+    P->time = 0.0;
+    P->converged = false;
+}
+
 int poly_set_freestream(PolyPath *P, std::string &err) {
     const uint32_t N = P->m.N;
+    reset_run(P);
     double U[5];
     prim_to_cons(P->prm.primvars_inf, U);
     for (int k = 0; k < 5; ++k) {
@@ -481,6 +510,7 @@ int poly_set_state(PolyPath *P, const double *U, const double *U_old, std::strin
         zero = false;
     }
     int rc;
+    reset_run(P);
     if ((rc = put_soa(P, P->U, U, 5, err))) return rc;
     hipLaunchKernelGGL(k_poly_prim, grid(N), dim3(PB), 0, P->st, (const double *)P->U, P->V, N, P->m.NT);
     if (zero)
@@ -514,13 +544,24 @@ int poly_calc_timestep(PolyPath *P, double *dt_out, std::string &err) {
 // Solver::step (src/Solver.cpp:34-52, 83-133)
 int poly_step(PolyPath *P, double dt, std::string &err) {
     int rc;
-    if ((rc = flux_balance(P, P->V, 0, dt, false, err))) return rc;
-    if (P->prm.time_scheme == NF_TVD_RK3) {
-        if ((rc = flux_balance(P, P->V, 1, dt, false, err))) return rc;
-        if ((rc = flux_balance(P, P->V, 2, dt, false, err))) return rc;
+    const bool rk3 = P->prm.time_scheme == NF_TVD_RK3;
+    if ((rc = flux_balance(P, P->V, 0, dt, false, !rk3, err))) return rc;
+    if (rk3) {
+        if ((rc = flux_balance(P, P->V, 1, dt, false, false, err))) return rc;
+        if ((rc = flux_balance(P, P->V, 2, dt, false, true, err))) return rc;
     }
     PHIP(hipMemcpyAsync(P->Uo, P->U, 5 * (size_t)P->m.N * 8, hipMemcpyDeviceToDevice, P->st));
+    // residual norm ||R_rho||_2 of the last stage
+    const uint32_t nb = (P->m.N + PB - 1) / PB;
+    hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, P->m.N, P->part);
+    PHIP(hipGetLastError());
+    std::vector<double> part(nb);
+    PHIP(hipMemcpyAsync(part.data(), P->part, nb * 8, hipMemcpyDeviceToHost, P->st));
     PHIP(hipStreamSynchronize(P->st));
+    double r2 = 0.0;
+    for (double x : part) r2 += x;
+    P->hist.push_back(std::sqrt(r2));
+    P->time += dt;
     return NF_OK;
 }
 
@@ -528,13 +569,62 @@ int poly_step(PolyPath *P, double dt, std::string &err) {
 // the primvars a following calc_timestep reads are left as the last step set them
 int poly_eval_flux_balance(PolyPath *P, std::string &err) {
     int rc;
-    if ((rc = flux_balance(P, P->Ve, -1, 0.0, true, err))) return rc;
+    if ((rc = flux_balance(P, P->Ve, -1, 0.0, true, false, err))) return rc;
     PHIP(hipStreamSynchronize(P->st));
     P->aux_valid = true;
     return NF_OK;
 }
 
 int poly_get_state(PolyPath *P, double *U, std::string &err) { return get_soa(P, P->U, 5, U, err); }
+
+// The Driver loop (calc_timestep, step) for n steps, stopping after the step
+// whose residual norm is at most drop times the first step's; a run that has
+// reached its drop stays stopped until the state is set again.
+int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err) {
+    if (!o || o->n_steps < 0) return NF_E_ARG;
+    const double drop = o->residual_drop;
+    hipEvent_t e0, e1;
+    PHIP(hipEventCreate(&e0));
+    PHIP(hipEventCreate(&e1));
+    PHIP(hipEventRecord(e0, P->st));
+    int done = 0, rc = NF_OK;
+    double dt = 0.0;
+    if (!(drop > 0.0 && P->converged)) {
+        while (done < o->n_steps) {
+            if ((rc = poly_calc_timestep(P, &dt, err)) || (rc = poly_step(P, dt, err))) break;
+            ++done;
+            if (drop > 0.0 && P->hist[0] > 0.0 && P->hist.back() / P->hist[0] <= drop) {
+                P->converged = true;
+                break;
+            }
+        }
+    }
+    float ms = 0.0f;
+    (void)hipEventRecord(e1, P->st);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        stats->steps_done = done;
+        stats->converged = P->converged ? 1 : 0;
+        stats->time = P->time;
+        stats->last_dt = dt;
+        stats->res_first = P->hist.empty() ? 0.0 : P->hist.front();
+        stats->res_last = P->hist.empty() ? 0.0 : P->hist.back();
+        stats->elapsed_ms = ms;
+        stats->fell_back = 0;
+        stats->pad = 0;
+    }
+    return rc;
+}
+
+int poly_get_residuals(PolyPath *P, double *out, int32_t max, int32_t *n_out) {
+    const int32_t n = (int32_t)std::min<size_t>(P->hist.size(), (size_t)std::max(max, 0));
+    std::copy(P->hist.begin(), P->hist.begin() + n, out);
+    if (n_out) *n_out = n;
+    return NF_OK;
+}
 
 int poly_get_field(PolyPath *P, int32_t which, double *out, std::string &err) {
     switch (which) {

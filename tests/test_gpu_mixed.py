@@ -102,9 +102,11 @@ def test_mixed_unsupported_calls_and_errors():
         nf.EulerSolver(sod_config(precision="mixed"), mesh)
     s = nf.EulerSolver(sod_config(), mesh)
     with pytest.raises(RuntimeError, match="mixed-element"):
-        s.run(5)
+        s.time_kernels(2)
     with pytest.raises(RuntimeError, match="mixed-element"):
         s.set_precision("mixed")
+    with pytest.raises(RuntimeError):
+        nf.BatchSolver([sod_config()] * 2, [mesh, mesh])
     # a non-physical state is reported with the reference's message
     U = sod_state(mesh)
     U[3, 0] = np.nan
@@ -150,3 +152,36 @@ def test_mixed_sim_dir_matches_oracle(tmp_path):
     np.testing.assert_array_equal(solver.get_solver_data().get_solution(), ora.get("U"))
     assert run_cfg.time == t
     assert len(os.listdir(d / "output")) == n_steps + 1
+
+
+def test_mixed_run_loop_and_residuals():
+    """nf_run on a mixed mesh is the Driver loop: the same state as stepping
+    by hand, the residual norm ||R_rho||_2 of each step's last stage (the
+    oracle's R after the step), and the residual-drop stop."""
+    mesh = mixed_box((5, 4, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=31)
+    cfg = sod_config()
+    U0 = sod_state(mesh)
+    a, ora = _pair(mesh, cfg, U0)
+    st = a.run(30)
+    assert st["steps_done"] == 30 and not st["converged"]
+    b = nf.EulerSolver(cfg, mesh)
+    b.set_state(U0)
+    for _ in range(30):
+        b.calc_timestep(cfg)
+        b.step(cfg)
+    np.testing.assert_array_equal(a.get_solver_data().get_solution(), b.get_solver_data().get_solution())
+    ora.run(30)
+    r = a.residuals()
+    assert len(r) == 30 and st["res_last"] == r[-1] and st["res_first"] == r[0]
+    assert r[-1] == pytest.approx(np.linalg.norm(ora.get("R")[:, 0]), rel=1e-13)
+    assert st["time"] == pytest.approx(ora.time(), rel=0, abs=0)
+    # stop at a drop of 0.5 of the first step's residual: a pulse leaving
+    # through farfield boundaries
+    ff = mixed_box((5, 4, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=31, patches=("FarField",) * 6)
+    a = nf.EulerSolver(pulse_config(), ff)
+    a.set_state(pulse_state(ff))
+    st = a.run(5000, residual_drop=0.5)
+    r = a.residuals()
+    assert st["converged"] and st["steps_done"] == len(r) < 5000
+    assert r[-1] / r[0] <= 0.5 < r[-2] / r[0]
+    assert a.run(10, residual_drop=0.5)["steps_done"] == 0   # stays stopped

@@ -164,3 +164,19 @@ def test_vtk_and_validity_mixed(tmp_path):
     i = lines.index(f"CELL_TYPES {N}")
     np.testing.assert_array_equal([int(x) for x in lines[i + 1:i + 1 + N]], ct)
     np.testing.assert_allclose(fe.read_vtk_cell_data(path)["density"], 1.0)
+
+
+def test_hex_box():
+    m = nf.Mesh.hex_box((4, 3, 2), lengths=(2.0, 1.5, 1.0))
+    g = m.fv_grid()
+    N, NT, Fint, F, P = g["sizes"]
+    assert (N, P) == (24, 6) and NT - N == F - Fint == 2 * (4 * 3 + 4 * 2 + 3 * 2)
+    np.testing.assert_allclose(g["volume"][:N], 0.125, rtol=1e-14)
+    np.testing.assert_array_equal(g["patch_nfaces"], [6, 6, 8, 8, 12, 12])
+    j = nf.Mesh.hex_box(5, jitter=0.2, seed=3, patches=("FarField",) * 6)
+    gj, go = j.fv_grid(), Oracle.from_mesh(j, _cfg()).grid()
+    for k in GRID_KEYS:
+        np.testing.assert_array_equal(gj[k], go[k], err_msg=k)
+    # non-planar faces: the six-pyramid volumes of neighbours do not tile the
+    # box exactly (each hexahedron splits its faces about its own centre)
+    assert np.all(gj["volume"][:125] > 0) and gj["volume"][:125].sum() == pytest.approx(1.0, rel=1e-3)
