@@ -491,24 +491,22 @@ struct GhostArgs {
     double *out;                 // [nb/64][5][64], tiled like V
     const FarField<double> *fs;  // [instances]
     int src_is_u;                // cell ghosts: src is U, the primvars are prim(U)
+    const double *bS;            // [3][nb] S_ij of each boundary face (cell -> ghost)
 };
 
 // FACE = false: ghost of the cell-centre state (set_constant_ghost_values,
 // src/Solver.cpp:300-330); FACE = true: ghost of the reconstructed face state
 // (src/Solver.cpp:210-212)
+// The face vector comes precomputed from the host face graph: the same
+// cross product and sign the sweeps evaluate from the nodes, so the same bits.
 template <bool FACE>
 __global__ __launch_bounds__(BLOCK) void k_ghost(DevMesh m, GhostArgs a) {
-    __shared__ double lds[12 * BLOCK];
     const uint32_t b0 = blockIdx.x * BLOCK + threadIdx.x;
     const bool live = b0 < a.nb;
     const uint32_t b = live ? b0 : a.nb - 1;
     const uint32_t inst = a.binst[b];
     const uint32_t w = a.bface[b], c = w & 0x1FFFFFFFu, s = w >> 29;
-    CellGeo g;
-    load_cell(m, c, g);
-    const NodeView nv = stage_nodes<BLOCK>(lds, g);
-    V3d S;
-    face_geo(m, g, nv, (m.fcode[c] >> (8 * s)) & 0xFFu, S, nullptr);
+    const V3d S = {a.bS[b], a.bS[a.nb + b], a.bS[2 * (size_t)a.nb + b]};
     double Vd[5], Vg[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vd[k] = FACE ? a.src[fix(c, s, k)] : a.src[tix(c, k)];

@@ -507,6 +507,7 @@ void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order, CellLayo
     L.fcode.assign(N, 0);
     L.bface.clear();
     L.bbc.clear();
+    L.bfidx.clear();
     for (uint32_t d = 0; d < N; ++d) {
         const uint32_t r = L.perm[d];
         const uint32_t *v = &m.cells[4 * (size_t)r];
@@ -531,6 +532,7 @@ void build_cell_layout(const PrimalMesh &m, const FVGrid &g, int order, CellLayo
                 w = NBR_BOUNDARY | (uint32_t)L.bface.size();
                 L.bface.push_back(d | (s << 29));
                 L.bbc.push_back((uint8_t)face_bc[f - g.Fint]);
+                L.bfidx.push_back(f);
                 L.ghost[4 * (size_t)d + s] = j;
             } else {
                 uint32_t other = (i == r) ? j : i;

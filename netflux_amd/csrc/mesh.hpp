@@ -69,6 +69,7 @@ struct CellLayout {
     std::vector<uint32_t> ghost;  // [N][4] reference ghost id (boundary slots)
     std::vector<uint32_t> bface;  // [nb] device cell | slot << 29, per boundary face
     std::vector<uint8_t> bbc;     // [nb] boundary type of that face
+    std::vector<uint32_t> bfidx;  // [nb] its reference face index
 };
 
 // neighbour word: bit31 = boundary.  interior: bits 29-30 neighbour slot,

@@ -229,6 +229,7 @@ struct nf_ctx {
     uint32_t nbf = 0;
     uint32_t *bface = nullptr;
     uint8_t *bbc = nullptr;
+    double *bS = nullptr;  // [3][nbf] S_ij of the boundary faces
     uint16_t *binst = nullptr;
     double *Gc = nullptr, *Gf = nullptr;
     float *Gc32 = nullptr, *Gf32 = nullptr;
@@ -349,7 +350,7 @@ int run_ghost(nf_ctx *ctx, bool face, const double *U = nullptr) {
                          [&] { return mx::launch_ghost_mx(face, g, ctx->stream, m, a); });
     }
     const GhostArgs a{ctx->bface, ctx->bbc, ctx->binst, ctx->nbf, face ? ctx->FV : (U ? U : ctx->V),
-                      face ? ctx->Gf : ctx->Gc, ctx->fs, (!face && U) ? 1 : 0};
+                      face ? ctx->Gf : ctx->Gc, ctx->fs, (!face && U) ? 1 : 0, ctx->bS};
     if (face) return LAUNCH("ghost_face", k_ghost<true>, g, b, m, a);
     return LAUNCH("ghost_cell", k_ghost<false>, g, b, m, a);
 }
@@ -792,6 +793,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     std::vector<uint32_t> cnodes, nbr, fcode, bface, inst_end;
     std::vector<float4> wall;
     std::vector<uint8_t> bbc;
+    std::vector<double> bS3;  // [nb][3]
     std::vector<uint16_t> binst;
     std::vector<uint32_t> blk_inst;
     std::vector<double> nodes4;
@@ -843,6 +845,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
             const uint32_t w = L.bface[q];
             bface.push_back((w & ~NBR_INDEX_MASK) | ((w & NBR_INDEX_MASK) + (uint32_t)cell0));
             bbc.push_back(L.bbc[q]);
+            for (int d = 0; d < 3; ++d) bS3.push_back(g.S[3 * (size_t)L.bfidx[q] + d]);
             binst.push_back((uint16_t)i);
         }
         // wall-face centroids (slip and no-slip patches) for the network features
@@ -927,6 +930,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     if (p0.local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NC))) return rc;
     if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
+    if ((rc = dalloc(ctx, &ctx->bS, 3 * (size_t)ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->binst, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->fs, (size_t)n))) return rc;
     if ((rc = dalloc(ctx, &ctx->fs32, (size_t)n))) return rc;
@@ -948,6 +952,10 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
     if (ctx->nbf) {
         HIPC(hipMemcpy(ctx->bface, bface.data(), bface.size() * 4, hipMemcpyHostToDevice));
         HIPC(hipMemcpy(ctx->bbc, bbc.data(), bbc.size(), hipMemcpyHostToDevice));
+        std::vector<double> bS((size_t)3 * ctx->nbf);
+        for (size_t q = 0; q < ctx->nbf; ++q)
+            for (int d = 0; d < 3; ++d) bS[d * (size_t)ctx->nbf + q] = bS3[3 * q + d];
+        HIPC(hipMemcpy(ctx->bS, bS.data(), bS.size() * 8, hipMemcpyHostToDevice));
         HIPC(hipMemcpy(ctx->binst, binst.data(), binst.size() * 2, hipMemcpyHostToDevice));
     }
     {
