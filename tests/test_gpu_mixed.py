@@ -196,3 +196,28 @@ def test_hex_box_24cubed_matches_oracle(flux):
     gpu, ora = _pair(mesh, cfg, sod_state(mesh))
     _steps(gpu, ora, cfg, 20)
     np.testing.assert_array_equal(gpu.get_solver_data().get_solution(), ora.get("U"))
+
+
+def test_single_pyramid_and_thin_mesh():
+    """Ragged extremes: one cell in a 256-lane block (a pyramid with a
+    farfield base and slip sides), and a one-block-wide column of wedges.
+    The single cell has no interior face, so its Delta S is zero and the
+    reference's time step is DBL_MAX (only interior faces count); its flux
+    balance is still well defined."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0.4, 0.6, 0.9]], float)
+    one = nf.Mesh.from_elements(nodes, [[0, 1, 2, 3, 4]], [[[0, 3, 2, 1]], [[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]]],
+                                ["FarField", "SlipWall"])
+    cfg = pulse_config(mach=0.4)
+    print("[case] single pyramid", flush=True)
+    gpu, ora = _pair(one, cfg, pulse_state(one, centre=(0.5, 0.5, 0.3)))
+    gpu.calc_timestep(cfg)
+    assert cfg.delta_time == ora.calc_timestep() == np.finfo(float).max
+    gpu.eval_flux_balance()
+    ora.eval_flux_balance()
+    R_g, R_o = gpu.get_solver_data().get_flux_balance(), ora.get("R")
+    assert np.all(np.isfinite(R_g)) and np.max(np.abs(R_g - R_o)) <= 1e-13 * np.max(np.abs(R_o))
+    thin = mixed_box((1, 1, 7), kinds=("wedge",), jitter=0.0, seed=2, patches=("FarField",) * 2 + ("SlipWall",) * 4)
+    print(f"[case] thin wedge column cells={thin.n_cells}", flush=True)
+    gpu, ora = _pair(thin, cfg, pulse_state(thin))
+    _steps(gpu, ora, cfg, 5, exact=False)
+    assert rel_l2(gpu.get_solver_data().get_solution(), ora.get("U")) <= TOL_REL_L2
