@@ -277,7 +277,8 @@ struct PolyPath {
     int *bad = nullptr;
     bool aux_valid = false;
     // run state since the last set_state / set_freestream
-    std::vector<double> hist;  // residual norm of each step's last stage
+    std::vector<uint32_t> perm;  // device cell -> reference cell
+    std::vector<double> hist;    // residual norm of each step's last stage
     double time = 0.0;
     bool converged = false;
     std::vector<void *> owned;
@@ -308,12 +309,12 @@ template <typename T> int upload(T *dst, const std::vector<T> &src, std::string 
 }
 dim3 grid(size_t n) { return dim3((unsigned)((n + PB - 1) / PB)); }
 
-// AoS [N][cols] <-> SoA [cols][N]
+// AoS [N][cols] in reference cell order <-> SoA [cols][N] in device order
 int put_soa(PolyPath *P, double *dst, const double *aos, int cols, std::string &err) {
     const uint32_t N = P->m.N;
     std::vector<double> soa((size_t)cols * N);
-    for (uint32_t c = 0; c < N; ++c)
-        for (int k = 0; k < cols; ++k) soa[k * (size_t)N + c] = aos[(size_t)cols * c + k];
+    for (uint32_t d = 0; d < N; ++d)
+        for (int k = 0; k < cols; ++k) soa[k * (size_t)N + d] = aos[(size_t)cols * P->perm[d] + k];
     PHIP(hipMemcpyAsync(dst, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, P->st));
     PHIP(hipStreamSynchronize(P->st));
     return NF_OK;
@@ -323,8 +324,8 @@ int get_soa(PolyPath *P, const double *src, int cols, double *aos, std::string &
     std::vector<double> soa((size_t)cols * N);
     PHIP(hipStreamSynchronize(P->st));
     PHIP(hipMemcpy(soa.data(), src, soa.size() * 8, hipMemcpyDeviceToHost));
-    for (uint32_t c = 0; c < N; ++c)
-        for (int k = 0; k < cols; ++k) aos[(size_t)cols * c + k] = soa[k * (size_t)N + c];
+    for (uint32_t d = 0; d < N; ++d)
+        for (int k = 0; k < cols; ++k) aos[(size_t)cols * P->perm[d] + k] = soa[k * (size_t)N + d];
     return NF_OK;
 }
 
@@ -397,6 +398,16 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
         err = "mesh too large for the mixed-element path";
         return NF_E_MESH;
     }
+    // device numbering (cell_order); each cell keeps its faces in increasing
+    // reference face index, so every sum keeps the reference's order
+    if (p.cell_order == NF_ORDER_RCM) P->perm = rcm_order(g);
+    else if (p.cell_order == NF_ORDER_MORTON) P->perm = morton_order(g);
+    else {
+        P->perm.resize(N);
+        for (uint32_t c = 0; c < N; ++c) P->perm[c] = c;
+    }
+    std::vector<uint32_t> iperm(N);
+    for (uint32_t d = 0; d < N; ++d) iperm[P->perm[d]] = d;
     // slots: the faces of each cell in increasing face index
     std::vector<uint32_t> w((size_t)6 * N, NONE);
     std::vector<uint8_t> ns((size_t)6 * N, 0), cnt(N, 0);
@@ -414,7 +425,8 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
         }
     };
     for (uint32_t f = 0; f < F; ++f) {
-        const uint32_t i = g.ij[2 * (size_t)f], j = g.ij[2 * (size_t)f + 1];
+        const uint32_t i = iperm[g.ij[2 * (size_t)f]], jr = g.ij[2 * (size_t)f + 1];
+        const uint32_t j = jr < N ? iperm[jr] : jr;  // ghosts keep their reference id
         if (cnt[i] >= 6 || (j < N && cnt[j] >= 6)) {
             err = "cell with more than six faces";
             return NF_E_MESH;
@@ -437,10 +449,11 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
         }
         put_geo(i, si, f, g.ri);
     }
-    std::vector<double> vol(g.vol.begin(), g.vol.begin() + N);
-    std::vector<double> dS((size_t)3 * N);
-    for (uint32_t c = 0; c < N; ++c)
-        for (int d = 0; d < 3; ++d) dS[d * (size_t)N + c] = g.dS[3 * (size_t)c + d];
+    std::vector<double> vol(N), dS((size_t)3 * N);
+    for (uint32_t c = 0; c < N; ++c) {
+        vol[c] = g.vol[P->perm[c]];
+        for (int d = 0; d < 3; ++d) dS[d * (size_t)N + c] = g.dS[3 * (size_t)P->perm[c] + d];
+    }
     uint32_t *dw, *dbi, *dbj;
     uint8_t *dns, *dbbc;
     double *dgeo, *dvol, *ddS, *dbS;
