@@ -153,7 +153,22 @@ def bench_hex(args, nf, pulse_config, pulse_state):
         solver.step(cfg)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    step_bytes = poly_bytes_model(N, F, F - Fint)
+    G = F - Fint
+    step_bytes = poly_bytes_model(N, F, G)
+    # per-kernel HIP-event durations on the solver's stream
+    kt = solver.time_kernels(args.timing_steps)
+    kern = {name: {"avg_ms": avg, "calls": calls} for name, (avg, calls) in kt.items()}
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["calls"])
+    slots = 2 * Fint + G
+    per_launch = {"poly_sweepA": 72 * N + 88 * slots, "poly_sweepB": 158 * N + 64 * slots}.get(dom)
+    roofline = None
+    if per_launch:
+        achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "traffic_note": "PMC upper bounds in profiles/poly_kernels_hex96.md",
+                    "kernel": dom, "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
+                    "algorithmic_bytes_per_launch": int(per_launch)}
     out = {
         "metric": "cell-stage-updates/s",
         "value": N * 3 * args.steps / wall,
@@ -166,7 +181,8 @@ def bench_hex(args, nf, pulse_config, pulse_state):
                    "cells": N, "faces": F, "timing": "wall clock, Driver loop (calc_timestep + step)"},
         "step_roofline_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "step_algorithmic_bytes": step_bytes,
-        "roofline": None,
+        "kernels": kern,
+        "roofline": roofline,
         "cpu_baseline": None,
     }
     print(json.dumps(out))

@@ -30,6 +30,8 @@ int poly_eval_flux_balance(PolyPath *P, std::string &err);
 int poly_get_state(PolyPath *P, double *U, std::string &err);
 int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err);
 int poly_get_residuals(PolyPath *P, double *out, int32_t max, int32_t *n_out);
+int poly_time_kernels(PolyPath *P, int32_t n_steps, nf_kernel_time *out, int32_t max, int32_t *n_out,
+                      std::string &err);
 int poly_get_field(PolyPath *P, int32_t which, double *out, std::string &err);
 
 }  // namespace nf

@@ -984,7 +984,7 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
 
 const char *POLY_UNSUPPORTED =
     "not available for a mixed-element mesh (supported: set_freestream, set_state, calc_timestep, step, run, "
-    "get_residuals, eval_flux_balance, get_state, get_field, ctx_sizes)";
+    "get_residuals, time_kernels, eval_flux_balance, get_state, get_field, ctx_sizes)";
 bool single(const nf_ctx *ctx) {
     if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
     return ctx && ctx->n_inst == 1;
@@ -1377,6 +1377,7 @@ int nf_get_field(nf_ctx *ctx, int32_t which, double *out) {
 }
 
 int nf_time_kernels(nf_ctx *ctx, int32_t n_steps, nf_kernel_time *out, int32_t max, int32_t *n_out) {
+    if (ctx && ctx->poly) return nf::poly_time_kernels(ctx->poly, n_steps, out, max, n_out, ctx->err);
     if (!single(ctx) || n_steps < 1 || !out) return NF_E_ARG;
     int rc;
     if (!ctx->inst[0].dt_valid) {

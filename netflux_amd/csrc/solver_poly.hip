@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <vector>
@@ -277,6 +278,10 @@ struct PolyPath {
     int *bad = nullptr;
     bool aux_valid = false;
     // run state since the last set_state / set_freestream
+    // kernel timing (nf_time_kernels): HIP events around each launch
+    bool timing = false;
+    std::vector<std::pair<const char *, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::map<std::string, std::pair<double, int>> ktime;
     std::vector<uint32_t> perm;  // device cell -> reference cell
     std::vector<double> hist;    // residual norm of each step's last stage
     double time = 0.0;
@@ -329,18 +334,55 @@ int get_soa(PolyPath *P, const double *src, int cols, double *aos, std::string &
     return NF_OK;
 }
 
+int tstart(PolyPath *P, hipEvent_t *e, std::string &err) {
+    if (!P->timing) return NF_OK;
+    PHIP(hipEventCreate(e));
+    PHIP(hipEventRecord(*e, P->st));
+    return NF_OK;
+}
+int tstop(PolyPath *P, const char *name, hipEvent_t e0, std::string &err) {
+    if (!P->timing) return NF_OK;
+    hipEvent_t e1;
+    PHIP(hipEventCreate(&e1));
+    PHIP(hipEventRecord(e1, P->st));
+    P->pending.push_back({name, {e0, e1}});
+    return NF_OK;
+}
+int tflush(PolyPath *P, std::string &err) {
+    PHIP(hipStreamSynchronize(P->st));
+    for (auto &q : P->pending) {
+        float ms = 0.0f;
+        PHIP(hipEventElapsedTime(&ms, q.second.first, q.second.second));
+        auto &slot = P->ktime[q.first];
+        slot.first += ms;
+        slot.second += 1;
+        (void)hipEventDestroy(q.second.first);
+        (void)hipEventDestroy(q.second.second);
+    }
+    P->pending.clear();
+    return NF_OK;
+}
+
 // evaluate_flux_balance of U into R (and the stage update when kind >= 0):
 // primvars into V, ghosts, sweep A for second order, sweep B
 int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, std::string &err) {
     const PolyDev &m = P->m;
     const bool check = P->prm.check_physical_validity != 0;
+    int rc;
+    hipEvent_t e = nullptr;
+    if ((rc = tstart(P, &e, err))) return rc;
     hipLaunchKernelGGL(k_poly_prim, grid(m.N), dim3(PB), 0, P->st, (const double *)P->U, V, m.N, m.NT);
+    if ((rc = tstop(P, "poly_prim", e, err)) || (rc = tstart(P, &e, err))) return rc;
     if (m.G) hipLaunchKernelGGL(k_poly_ghost, grid(m.G), dim3(PB), 0, P->st, m, V, P->fs);
+    if ((rc = tstop(P, "poly_ghost", e, err))) return rc;
     const bool second = P->prm.spatial_order == NF_SECOND_ORDER;
-    if (second)
+    if (second) {
+        if ((rc = tstart(P, &e, err))) return rc;
         hipLaunchKernelGGL(k_poly_sweepA, grid(m.N), dim3(PB), 0, P->st, m, (const double *)V,
                            P->prm.limiter != NF_LIMITER_NONE ? 1 : 0, P->FV, aux ? P->grad : nullptr,
                            aux ? P->psi : nullptr);
+        if ((rc = tstop(P, "poly_sweepA", e, err))) return rc;
+    }
     StageArgs a;
     a.V = V;
     a.FV = second ? P->FV : nullptr;
@@ -354,8 +396,10 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res
     a.dt = dt;
     a.fs = P->fs;
     if (check) PHIP(hipMemsetAsync(P->bad, 0, sizeof(int), P->st));
+    if ((rc = tstart(P, &e, err))) return rc;
     hipLaunchKernelGGL(k_poly_sweepB, grid(m.N), dim3(PB), 0, P->st, m, a);
     PHIP(hipGetLastError());
+    if ((rc = tstop(P, "poly_sweepB", e, err))) return rc;
     if (check) {
         int nbad = 0;
         PHIP(hipMemcpyAsync(&nbad, P->bad, sizeof(int), hipMemcpyDeviceToHost, P->st));
@@ -630,6 +674,33 @@ int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string
         stats->pad = 0;
     }
     return rc;
+}
+
+// HIP-event durations of each kernel over n steps of the Driver loop
+int poly_time_kernels(PolyPath *P, int32_t n_steps, nf_kernel_time *out, int32_t max, int32_t *n_out,
+                      std::string &err) {
+    if (n_steps < 1 || !out) return NF_E_ARG;
+    P->ktime.clear();
+    P->timing = true;
+    int rc = NF_OK;
+    for (int q = 0; q < n_steps && rc == NF_OK; ++q) {
+        double dt;
+        if ((rc = poly_calc_timestep(P, &dt, err)) == NF_OK) rc = poly_step(P, dt, err);
+        const int f = tflush(P, err);
+        if (rc == NF_OK) rc = f;
+    }
+    P->timing = false;
+    if (rc) return rc;
+    int n = 0;
+    for (auto &kv : P->ktime) {
+        if (n >= max) break;
+        std::snprintf(out[n].name, sizeof out[n].name, "%s", kv.first.c_str());
+        out[n].avg_ms = kv.second.first / kv.second.second;
+        out[n].calls = kv.second.second;
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    return NF_OK;
 }
 
 int poly_get_residuals(PolyPath *P, double *out, int32_t max, int32_t *n_out) {

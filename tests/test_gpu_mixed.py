@@ -102,7 +102,10 @@ def test_mixed_unsupported_calls_and_errors():
         nf.EulerSolver(sod_config(precision="mixed"), mesh)
     s = nf.EulerSolver(sod_config(), mesh)
     with pytest.raises(RuntimeError, match="mixed-element"):
-        s.time_kernels(2)
+        s.autotune(2)
+    kt = s.time_kernels(2)   # HIP-event durations of the polyhedral kernels
+    assert set(kt) == {"poly_prim", "poly_ghost", "poly_sweepA", "poly_sweepB"}
+    assert all(calls == 6 and avg > 0 for avg, calls in kt.values())
     with pytest.raises(RuntimeError, match="mixed-element"):
         s.set_precision("mixed")
     with pytest.raises(RuntimeError):
