@@ -225,3 +225,43 @@ def test_single_pyramid_and_thin_mesh():
     gpu, ora = _pair(thin, cfg, pulse_state(thin))
     _steps(gpu, ora, cfg, 5, exact=False)
     assert rel_l2(gpu.get_solver_data().get_solution(), ora.get("U")) <= TOL_REL_L2
+
+
+@pytest.mark.parametrize("name,n_steps", [("cyprien_hex_tutorial", 20), ("mixed_tiny", 12)])
+def test_reference_hex_and_mixed_sim_dirs(tmp_path, name, n_steps):
+    """The reference's hexahedral tutorial and mixed-element input.yml files
+    (their meshes are not shipped) over generated SU2 meshes of that kind,
+    run unchanged through the front end and checked against the oracle."""
+    cfg, opts = fe.parse_input(os.path.join(GOLD, name + ".yml"))
+    names = list(opts["patches"])
+    if name == "cyprien_hex_tutorial":
+        m = nf.Mesh.hex_box((8, 4, 4), lengths=(2.0, 1.0, 1.0), jitter=0.15, seed=7)
+        groups = [[0, 1, 2, 3, 4, 5]]
+    else:
+        m = mixed_box((6, 3, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=9, lengths=(2.0, 1.0, 1.0))
+        groups = [[0], [1], [2, 3, 4, 5]]
+    ct, co, cn, pn, ft, fo, fn, pbc = m.elements()
+    faces, e = [], 0
+    for k in pn:
+        faces.append([fn[fo[q]:fo[q + 1]] for q in range(e, e + k)])
+        e += k
+    X = np.ctypeslib.as_array(m.desc.nodes, shape=(m.desc.n_nodes, 3)).copy()
+    mesh = nf.Mesh.from_elements(X, [cn[co[q]:co[q + 1]] for q in range(len(ct))],
+                                 [sum((faces[g] for g in grp), []) for grp in groups], ["SlipWall"] * len(groups))
+    d = tmp_path / name
+    d.mkdir()
+    text = open(os.path.join(GOLD, name + ".yml")).read()
+    d.joinpath("input.yml").write_text("\n".join(f"n_timesteps: {n_steps}" if l.startswith("n_timesteps:") else l
+                                                 for l in text.split("\n")))
+    mesh.write_su2(d / "mesh.su2", names)
+    print(f"[case] reference sim dir {name} on a {'hexahedral' if len(groups) == 1 else 'mixed'} mesh, "
+          f"{n_steps} steps", flush=True)
+    solver, run_cfg = fe.run_sim_dir(str(d), log=lambda *_: None)
+    cfg.n_timesteps = n_steps
+    ora = Oracle.from_mesh(nf.Mesh.read_su2(d / "mesh.su2", opts["patches"]), cfg)
+    ora.set_freestream()
+    for _ in range(n_steps):
+        ora.step(ora.calc_timestep())
+    np.testing.assert_array_equal(solver.get_solver_data().get_solution(), ora.get("U"))
+    stride = opts["write_stride"]
+    assert sorted(os.listdir(d / "output")) == sorted(f"n={k}.vtk" for k in range(0, n_steps + 1, stride))
