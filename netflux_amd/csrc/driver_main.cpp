@@ -1,16 +1,20 @@
 // netflux_driver: runs NetFlux's time loop on the GPU through the C++ host
-// classes (netflux_solver.hpp), on a generated box mesh.
+// classes (netflux_solver.hpp), on a generated box mesh or an SU2 file.
 //
-//   netflux_driver <n> <steps> [sod|pulse] [state_out.bin]
+//   netflux_driver <n | su2:path> <steps> [sod|pulse] [state_out.bin]
 //
 // sod:   unit cube n^3 hexes x 6 tets, slip walls, Sod tube along x
 // pulse: same mesh jittered by 15%, farfield, Mach 0.5 + Gaussian pulse
+// su2:path reads that mesh (any element types; every marker a slip wall)
+// and applies the Sod state by cell centroid.
 // Prints one JSON line (cells, steps, time, sum of rho*vol); with a fourth
 // argument writes U (AoS [N][5] doubles, reference cell order) to that file.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -18,7 +22,7 @@
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s <n> <steps> [sod|pulse] [state_out.bin]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <n | su2:path> <steps> [sod|pulse] [state_out.bin]\n", argv[0]);
         return 2;
     }
     const uint32_t n = (uint32_t)std::atoi(argv[1]);
@@ -30,7 +34,26 @@ int main(int argc, char **argv) {
     const int32_t bc_ff[6] = {NF_BC_FARFIELD, NF_BC_FARFIELD, NF_BC_FARFIELD,
                               NF_BC_FARFIELD, NF_BC_FARFIELD, NF_BC_FARFIELD};
     nf_mesh *mesh = nullptr;
-    if (nf_mesh_box(n, n, n, 1, 1, 1, pulse ? 0.15 : 0.0, 20260101ull, 0, 0.2, pulse ? bc_ff : bc_slip, &mesh)) {
+    const std::string arg1 = argv[1];
+    if (arg1.compare(0, 4, "su2:") == 0) {
+        // marker tags as they appear in the file, each a slip wall
+        std::ifstream is(arg1.substr(4));
+        std::vector<std::string> tags;
+        for (std::string tok; is >> tok;) {
+            if (tok.compare(0, 11, "MARKER_TAG=") != 0) continue;
+            std::string name = tok.substr(11);
+            if (name.empty()) is >> name;
+            tags.push_back(name);
+        }
+        std::vector<const char *> names;
+        for (const std::string &t : tags) names.push_back(t.c_str());
+        const std::vector<int32_t> bcs(tags.size(), NF_BC_SLIP_WALL);
+        if (nf_mesh_read_su2(arg1.c_str() + 4, (int32_t)names.size(), names.data(), bcs.data(), &mesh)) {
+            std::fprintf(stderr, "%s\n", nf_mesh_last_error());
+            return 1;
+        }
+    } else if (nf_mesh_box(n, n, n, 1, 1, 1, pulse ? 0.15 : 0.0, 20260101ull, 0, 0.2, pulse ? bc_ff : bc_slip,
+                           &mesh)) {
         std::fprintf(stderr, "mesh generation failed\n");
         return 1;
     }

@@ -265,3 +265,22 @@ def test_reference_hex_and_mixed_sim_dirs(tmp_path, name, n_steps):
     np.testing.assert_array_equal(solver.get_solver_data().get_solution(), ora.get("U"))
     stride = opts["write_stride"]
     assert sorted(os.listdir(d / "output")) == sorted(f"n={k}.vtk" for k in range(0, n_steps + 1, stride))
+
+
+def test_cpp_driver_on_mixed_su2(tmp_path):
+    """The C++ host classes (netflux_solver.hpp) on a mixed-element SU2 file:
+    netflux::Driver with the Sod state, against the oracle."""
+    import subprocess
+    mesh = mixed_box((5, 3, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=13)
+    path = tmp_path / "mesh.su2"
+    mesh.write_su2(path, ["xm", "xp", "ym", "yp", "zm", "zp"])
+    exe = os.path.join(os.path.dirname(nf.LIB_PATH), "netflux_driver")
+    out = tmp_path / "U.bin"
+    print("[case] netflux_driver su2 mixed 30 sod", flush=True)
+    r = subprocess.run([exe, f"su2:{path}", "30", "sod", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = nf.Mesh.read_su2(path, {n: "SlipWall" for n in ["xm", "xp", "ym", "yp", "zm", "zp"]})
+    ora = Oracle.from_mesh(m, sod_config())
+    ora.set_state(sod_state(m), False)
+    ora.run(30)
+    np.testing.assert_array_equal(np.fromfile(out, dtype=np.float64).reshape(-1, 5), ora.get("U"))
