@@ -157,13 +157,12 @@ int nf_fv_export(const nf_mesh *mesh,
  * Replaces EulerSolver (include/Solver.hpp:52-79): constructed from the mesh
  * and config, stepped with calc_timestep + step in the Driver loop order
  * (src/Driver.cpp:31-57).  A mixed-element mesh (hexahedra, wedges,
- * pyramids) runs on the polyhedral FP64 path: one mesh, global time step,
- * and the calls set_freestream, set_state, calc_timestep, step, run (a host
- * loop of calc_timestep + step; check_every and use_graph are ignored),
- * get_residuals, time_kernels, eval_flux_balance, get_state, get_field and
- * ctx_sizes; the
- * others return NF_E_ARG with a message, and nf_create rejects
- * NF_PREC_MIXED and local_time_step for such a mesh. */
+ * pyramids) runs on the polyhedral FP64 path: one mesh, global or local time
+ * steps, and the calls set_freestream, set_state, calc_timestep, step, run
+ * (a host loop of calc_timestep + step; check_every and use_graph are
+ * ignored), get_residuals, time_kernels, eval_flux_balance, get_state,
+ * get_field and ctx_sizes; the others return NF_E_ARG with a message, and
+ * nf_create rejects NF_PREC_MIXED for such a mesh. */
 typedef struct nf_ctx nf_ctx;
 
 int nf_create(const nf_mesh_desc *mesh, const nf_params *params, nf_ctx **out);

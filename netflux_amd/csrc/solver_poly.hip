@@ -156,6 +156,7 @@ struct StageArgs {
     int flux;    // 0 Rusanov, 1 HLLC
     int kind;    // 0: U + dt/vol R, 1: RK3 stage 2, 2: RK3 stage 3, -1: no update
     double dt;
+    const double *dtc;  // per-cell steps (local time stepping), or null: dt
     FarField<double> fs;
 };
 
@@ -198,22 +199,22 @@ __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
     if (a.Rrho) a.Rrho[c] = R[0];
     if (!ok && a.bad) atomicAdd(a.bad, 1);
     if (a.kind < 0) return;
-    const double vol = m.vol[c];
+    const double vol = m.vol[c], dt = a.dtc ? a.dtc[c] : a.dt;
     for (int k = 0; k < 5; ++k) {
         const size_t q = k * N + c;
         const double u = a.U[q];
-        if (a.kind == 0) a.U[q] = u + a.dt / vol * R[k];
-        else if (a.kind == 1) a.U[q] = 3.0 / 4.0 * a.Uo[q] + 1.0 / 4.0 * u + 1.0 / 4.0 * a.dt / vol * R[k];
-        else a.U[q] = 1.0 / 3.0 * a.Uo[q] + 2.0 / 3.0 * u + 2.0 / 3.0 * a.dt / vol * R[k];
+        if (a.kind == 0) a.U[q] = u + dt / vol * R[k];
+        else if (a.kind == 1) a.U[q] = 3.0 / 4.0 * a.Uo[q] + 1.0 / 4.0 * u + 1.0 / 4.0 * dt / vol * R[k];
+        else a.U[q] = 1.0 / 3.0 * a.Uo[q] + 2.0 / 3.0 * u + 2.0 / 3.0 * dt / vol * R[k];
     }
 }
 
 // the stage update alone, after the host has checked the flux balance
 __global__ __launch_bounds__(PB) void k_poly_update(PolyDev m, double *U, const double *Uo, const double *R,
-                                                    int kind, double dt) {
+                                                    int kind, double dt_all, const double *dtc) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     if (c >= m.N) return;
-    const double vol = m.vol[c];
+    const double vol = m.vol[c], dt = dtc ? dtc[c] : dt_all;
     for (int k = 0; k < 5; ++k) {
         const size_t q = k * (size_t)m.N + c;
         const double u = U[q];
@@ -224,7 +225,8 @@ __global__ __launch_bounds__(PB) void k_poly_update(PolyDev m, double *U, const 
 }
 
 // calc_timestep (src/Solver.cpp:232-274): per-block minimum of the cell steps
-__global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, double CFL, double *part) {
+__global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, double CFL, double *part,
+                                                double *dt_cell) {
     __shared__ double sh[PB];
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     double t = DBL_MAX;
@@ -236,6 +238,7 @@ __global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, doub
         const double sy = (std::fabs(v[2]) + cs) * m.dS[(size_t)m.N + c];
         const double sz = (std::fabs(v[3]) + cs) * m.dS[2 * (size_t)m.N + c];
         const double x = CFL * m.vol[c] / ((sx + sy) + sz);
+        if (dt_cell) dt_cell[c] = x;  // local time stepping: each cell's own CFL step
         t = smin(t, x);
     }
     sh[threadIdx.x] = t;
@@ -275,6 +278,7 @@ struct PolyPath {
     double *U = nullptr, *Uo = nullptr, *V = nullptr, *Ve = nullptr, *FV = nullptr;
     double *grad = nullptr, *psi = nullptr, *R = nullptr, *part = nullptr;
     double *Rrho = nullptr;
+    double *dtc = nullptr;  // [N] per-cell steps (local time stepping)
     int *bad = nullptr;
     bool aux_valid = false;
     // run state since the last set_state / set_freestream
@@ -283,7 +287,7 @@ struct PolyPath {
     std::vector<std::pair<const char *, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::map<std::string, std::pair<double, int>> ktime;
     std::vector<uint32_t> perm;  // device cell -> reference cell
-    std::vector<double> hist;    // residual norm of each step's last stage
+    std::vector<double> hist;    // residual norm of each step (flux balance of U^n)
     double time = 0.0;
     bool converged = false;
     std::vector<void *> owned;
@@ -365,7 +369,9 @@ int tflush(PolyPath *P, std::string &err) {
 
 // evaluate_flux_balance of U into R (and the stage update when kind >= 0):
 // primvars into V, ghosts, sweep A for second order, sweep B
-int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, std::string &err) {
+// lts_dt: also store each cell's CFL step of these primvars (first stage of a
+// locally time-stepped step); the update then uses the per-cell steps
+int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, bool lts_dt, std::string &err) {
     const PolyDev &m = P->m;
     const bool check = P->prm.check_physical_validity != 0;
     int rc;
@@ -375,6 +381,10 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res
     if ((rc = tstop(P, "poly_prim", e, err)) || (rc = tstart(P, &e, err))) return rc;
     if (m.G) hipLaunchKernelGGL(k_poly_ghost, grid(m.G), dim3(PB), 0, P->st, m, V, P->fs);
     if ((rc = tstop(P, "poly_ghost", e, err))) return rc;
+    const bool lts = P->prm.local_time_step != 0 && kind >= 0;
+    if (lts_dt)
+        hipLaunchKernelGGL(k_poly_dt, grid(m.N), dim3(PB), 0, P->st, m, (const double *)V, P->prm.CFL, P->part,
+                           P->dtc);
     const bool second = P->prm.spatial_order == NF_SECOND_ORDER;
     if (second) {
         if ((rc = tstart(P, &e, err))) return rc;
@@ -394,6 +404,7 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res
     a.flux = P->prm.inv_flux_scheme == NF_FLUX_HLLC ? 1 : 0;
     a.kind = check ? -1 : kind;
     a.dt = dt;
+    a.dtc = lts ? P->dtc : nullptr;
     a.fs = P->fs;
     if (check) PHIP(hipMemsetAsync(P->bad, 0, sizeof(int), P->st));
     if ((rc = tstart(P, &e, err))) return rc;
@@ -411,7 +422,7 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res
         }
         if (kind >= 0) {
             hipLaunchKernelGGL(k_poly_update, grid(m.N), dim3(PB), 0, P->st, m, P->U, (const double *)P->Uo,
-                               (const double *)P->R, kind, dt);
+                               (const double *)P->R, kind, dt, (const double *)a.dtc);
             PHIP(hipGetLastError());
         }
     }
@@ -425,10 +436,6 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
     *out = nullptr;
     if (p.precision != NF_PREC_FP64) {
         err = "a mixed-element mesh runs in FP64 only";
-        return NF_E_ARG;
-    }
-    if (p.local_time_step) {
-        err = "a mixed-element mesh takes the global time step only";
         return NF_E_ARG;
     }
     std::unique_ptr<PolyPath> hold(new PolyPath);
@@ -517,7 +524,7 @@ int poly_create(const PrimalMesh &pm, const FVGrid &g, const nf_params &p, hipSt
         (rc = alloc(P, &P->FV, 30 * (size_t)N, err)) || (rc = alloc(P, &P->grad, 15 * (size_t)N, err)) ||
         (rc = alloc(P, &P->psi, 5 * (size_t)N, err)) || (rc = alloc(P, &P->R, 5 * (size_t)N, err)) ||
         (rc = alloc(P, &P->part, (N + PB - 1) / PB, err)) || (rc = alloc(P, &P->bad, 1, err)) ||
-        (rc = alloc(P, &P->Rrho, N, err)))
+        (rc = alloc(P, &P->Rrho, N, err)) || (rc = alloc(P, &P->dtc, N, err)))
         return rc;
     if ((rc = poly_set_freestream(P, err))) return rc;
     *out = hold.release();
@@ -581,7 +588,8 @@ int poly_set_state(PolyPath *P, const double *U, const double *U_old, std::strin
 
 int poly_calc_timestep(PolyPath *P, double *dt_out, std::string &err) {
     const uint32_t nb = (P->m.N + PB - 1) / PB;
-    hipLaunchKernelGGL(k_poly_dt, dim3(nb), dim3(PB), 0, P->st, P->m, (const double *)P->V, P->prm.CFL, P->part);
+    hipLaunchKernelGGL(k_poly_dt, dim3(nb), dim3(PB), 0, P->st, P->m, (const double *)P->V, P->prm.CFL, P->part,
+                       (double *)nullptr);
     PHIP(hipGetLastError());
     std::vector<double> part(nb);
     PHIP(hipMemcpyAsync(part.data(), P->part, nb * 8, hipMemcpyDeviceToHost, P->st));
@@ -601,14 +609,15 @@ int poly_calc_timestep(PolyPath *P, double *dt_out, std::string &err) {
 // Solver::step (src/Solver.cpp:34-52, 83-133)
 int poly_step(PolyPath *P, double dt, std::string &err) {
     int rc;
-    const bool rk3 = P->prm.time_scheme == NF_TVD_RK3;
-    if ((rc = flux_balance(P, P->V, 0, dt, false, !rk3, err))) return rc;
+    const bool rk3 = P->prm.time_scheme == NF_TVD_RK3, lts = P->prm.local_time_step != 0;
+    // the residual is the flux balance of U^n (first stage), as in the batched path
+    if ((rc = flux_balance(P, P->V, 0, dt, false, true, lts, err))) return rc;
     if (rk3) {
-        if ((rc = flux_balance(P, P->V, 1, dt, false, false, err))) return rc;
-        if ((rc = flux_balance(P, P->V, 2, dt, false, true, err))) return rc;
+        if ((rc = flux_balance(P, P->V, 1, dt, false, false, false, err))) return rc;
+        if ((rc = flux_balance(P, P->V, 2, dt, false, false, false, err))) return rc;
     }
     PHIP(hipMemcpyAsync(P->Uo, P->U, 5 * (size_t)P->m.N * 8, hipMemcpyDeviceToDevice, P->st));
-    // residual norm ||R_rho||_2 of the last stage
+    // residual norm ||R_rho||_2 of the first stage
     const uint32_t nb = (P->m.N + PB - 1) / PB;
     hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, P->m.N, P->part);
     PHIP(hipGetLastError());
@@ -626,7 +635,7 @@ int poly_step(PolyPath *P, double dt, std::string &err) {
 // the primvars a following calc_timestep reads are left as the last step set them
 int poly_eval_flux_balance(PolyPath *P, std::string &err) {
     int rc;
-    if ((rc = flux_balance(P, P->Ve, -1, 0.0, true, false, err))) return rc;
+    if ((rc = flux_balance(P, P->Ve, -1, 0.0, true, false, false, err))) return rc;
     PHIP(hipStreamSynchronize(P->st));
     P->aux_valid = true;
     return NF_OK;
