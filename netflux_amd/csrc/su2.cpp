@@ -153,7 +153,7 @@ void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<s
     if (!f) throw MeshError("Couldn't open " + path + " for writing");
     std::fprintf(f, "NDIME= 3\nNELEM= %u\n", m.n_cells());
     for (uint32_t e = 0; e < m.n_cells(); ++e) {
-        const int t = m.mixed() ? m.ctype[e] : ELEM_TET;
+        const int t = m.mixed() ? (int)m.ctype[e] : (int)ELEM_TET;
         const uint32_t *src = m.mixed() ? &m.cells[m.coff[e]] : &m.cells[4 * (size_t)e];
         uint32_t n[8];
         const int nn = elem_n_nodes(t);
@@ -172,7 +172,7 @@ void write_su2(const std::string &path, const PrimalMesh &m, const std::vector<s
     for (size_t q = 0; q < m.patch_n.size(); ++q) {
         std::fprintf(f, "MARKER_TAG= %s\nMARKER_ELEMS= %u\n", names[q].c_str(), m.patch_n[q]);
         for (uint32_t k = 0; k < m.patch_n[q]; ++k, ++t) {
-            const int ty = m.mixed() ? m.ftype[t] : ELEM_TRI;
+            const int ty = m.mixed() ? (int)m.ftype[t] : (int)ELEM_TRI;
             const uint32_t *src = m.mixed() ? &m.patch_tri[m.foff[t]] : &m.patch_tri[3 * t];
             std::fprintf(f, "%d", ty);
             for (int r = 0; r < elem_n_nodes(ty); ++r) std::fprintf(f, " %u", src[r]);
