@@ -180,3 +180,39 @@ def test_hex_box():
     # non-planar faces: the six-pyramid volumes of neighbours do not tile the
     # box exactly (each hexahedron splits its faces about its own centre)
     assert np.all(gj["volume"][:125] > 0) and gj["volume"][:125].sum() == pytest.approx(1.0, rel=1e-3)
+
+
+def test_mixed_descriptor_validation():
+    """nf_mesh_from_desc rejects inconsistent element arrays with a message."""
+    import ctypes as C
+    from netflux_amd import _abi as A
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0.5, 0.5, 1.0]], float)
+    conn = np.array([0, 1, 2, 3, 4], np.uint32)
+    fconn = np.array([0, 3, 2, 1, 0, 1, 4, 1, 2, 4, 2, 3, 4, 3, 0, 4], np.uint32)
+    pn = np.array([5], np.uint32)
+    pbc = np.array([1], np.int32)
+
+    def make(ctype, coff, ftype, foff):
+        keep = [np.ascontiguousarray(a) for a in (ctype, coff, ftype, foff)]
+        d = A.MeshDesc(len(nodes), nodes.ctypes.data_as(A.f64p), len(keep[0]), conn.ctypes.data_as(A.u32p), 1,
+                       pn.ctypes.data_as(A.u32p), fconn.ctypes.data_as(A.u32p), pbc.ctypes.data_as(A.i32p),
+                       keep[0].ctypes.data_as(A.u8p), keep[1].ctypes.data_as(A.u32p),
+                       keep[2].ctypes.data_as(A.u8p), keep[3].ctypes.data_as(A.u32p))
+        h = C.c_void_p()
+        rc = A.lib().nf_mesh_from_desc(C.byref(d), C.byref(h))
+        if rc == A.NF_OK:
+            A.lib().nf_mesh_destroy(h)
+        return rc, A.lib().nf_mesh_last_error().decode()
+
+    ok = (np.array([14], np.uint8), np.array([0, 5], np.uint32), np.array([9, 5, 5, 5, 5], np.uint8),
+          np.array([0, 4, 7, 10, 13, 16], np.uint32))
+    assert make(*ok)[0] == A.NF_OK
+    bad = [
+        ((np.array([12], np.uint8),) + ok[1:], "node count does not match"),           # hexahedron with 5 nodes
+        ((np.array([5], np.uint8),) + ok[1:], "not a volume element"),                 # a triangle as a cell
+        (ok[:2] + (np.array([9, 5, 5, 5, 10], np.uint8), ok[3]), "not a face type"),   # a tetrahedron as a face
+        (ok[:3] + (np.array([0, 4, 7, 10, 13, 15], np.uint32),), "node count does not match"),
+    ]
+    for args, msg in bad:
+        rc, err = make(*args)
+        assert rc == A.NF_E_MESH and msg in err, (rc, err)
