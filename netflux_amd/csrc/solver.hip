@@ -208,7 +208,7 @@ struct RedTable {
 
 struct nf_ctx {
     nf_params prm{};
-    std::string err;
+    mutable std::string err;  // last error message (also set by const queries)
     int device = 0;
     hipStream_t stream = nullptr;
     int n_inst = 0;
@@ -986,11 +986,11 @@ const char *POLY_UNSUPPORTED =
     "not available for a mixed-element mesh (supported: set_freestream, set_state, calc_timestep, step, run, "
     "get_residuals, time_kernels, eval_flux_balance, get_state, get_field, ctx_sizes)";
 bool single(const nf_ctx *ctx) {
-    if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
+    if (ctx && ctx->poly) ctx->err = POLY_UNSUPPORTED;
     return ctx && ctx->n_inst == 1;
 }
 bool valid_inst(const nf_ctx *ctx, int32_t i) {
-    if (ctx && ctx->poly) const_cast<nf_ctx *>(ctx)->err = POLY_UNSUPPORTED;
+    if (ctx && ctx->poly) ctx->err = POLY_UNSUPPORTED;
     return ctx && i >= 0 && i < ctx->n_inst;
 }
 
