@@ -45,16 +45,20 @@ struct PolyDev {
     const uint32_t *bi, *bj;
     const double *bS;
     const uint8_t *bbc;
+    const int *stop;  // device-resident run: every kernel returns once it is set (null: no check)
 };
 constexpr uint32_t BND = 0x40000000u, IDX = 0x3FFFFFFFu;
+
+__device__ __forceinline__ bool stopped(const int *stop) { return stop && *stop; }
 
 __device__ __forceinline__ double geo(const PolyDev &m, int s, int q, uint32_t c) {
     return m.geo[((size_t)s * 6 + q) * m.N + c];
 }
 
-__global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, uint32_t N, uint32_t NT) {
+__global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, uint32_t N, uint32_t NT,
+                                                  const int *stop) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
-    if (c >= N) return;
+    if (c >= N || stopped(stop)) return;
     double u[5], v[5];
     for (int k = 0; k < 5; ++k) u[k] = U[k * (size_t)N + c];
     cons_to_prim(u, v);
@@ -64,7 +68,7 @@ __global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, ui
 // set_constant_ghost_values (src/Solver.cpp:300-330)
 __global__ __launch_bounds__(PB) void k_poly_ghost(PolyDev m, double *V, FarField<double> fs) {
     const uint32_t b = blockIdx.x * PB + threadIdx.x;
-    if (b >= m.G) return;
+    if (b >= m.G || stopped(m.stop)) return;
     const uint32_t i = m.bi[b], j = m.bj[b];
     double vd[5], vg[5];
     for (int k = 0; k < 5; ++k) vd[k] = V[k * (size_t)m.NT + i];
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(PB) void k_poly_ghost(PolyDev m, double *V, FarFiel
 __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, int limiter, double *FV,
                                                     double *grad, double *psi) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
-    if (c >= m.N) return;
+    if (c >= m.N || stopped(m.stop)) return;
     const size_t NT = m.NT, N = m.N;
     double vc[5];
     for (int k = 0; k < 5; ++k) vc[k] = V[k * NT + c];
@@ -156,7 +160,8 @@ struct StageArgs {
     int flux;    // 0 Rusanov, 1 HLLC
     int kind;    // 0: U + dt/vol R, 1: RK3 stage 2, 2: RK3 stage 3, -1: no update
     double dt;
-    const double *dtc;  // per-cell steps (local time stepping), or null: dt
+    const double *dtp;  // the step's dt in device memory (device-resident run), or null: dt
+    const double *dtc;  // per-cell steps (local time stepping), or null
     FarField<double> fs;
 };
 
@@ -164,7 +169,7 @@ struct StageArgs {
 // check (src/SolverData.cpp:66-79) and the stage update (src/Solver.cpp:83-133)
 __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
-    if (c >= m.N) return;
+    if (c >= m.N || stopped(m.stop)) return;
     const size_t NT = m.NT, N = m.N;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < 6; ++s) {
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
     if (a.Rrho) a.Rrho[c] = R[0];
     if (!ok && a.bad) atomicAdd(a.bad, 1);
     if (a.kind < 0) return;
-    const double vol = m.vol[c], dt = a.dtc ? a.dtc[c] : a.dt;
+    const double vol = m.vol[c], dt = a.dtc ? a.dtc[c] : (a.dtp ? *a.dtp : a.dt);
     for (int k = 0; k < 5; ++k) {
         const size_t q = k * N + c;
         const double u = a.U[q];
@@ -228,6 +233,7 @@ __global__ __launch_bounds__(PB) void k_poly_update(PolyDev m, double *U, const 
 __global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, double CFL, double *part,
                                                 double *dt_cell) {
     __shared__ double sh[PB];
+    if (stopped(m.stop)) return;  // the same for every thread of the block
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     double t = DBL_MAX;
     if (c < m.N) {
@@ -251,8 +257,9 @@ __global__ __launch_bounds__(PB) void k_poly_dt(PolyDev m, const double *V, doub
 }
 
 // per-block sums of x^2 (the residual norm), fixed order
-__global__ __launch_bounds__(PB) void k_poly_sumsq(const double *x, uint32_t n, double *part) {
+__global__ __launch_bounds__(PB) void k_poly_sumsq(const double *x, uint32_t n, double *part, const int *stop) {
     __shared__ double sh[PB];
+    if (stopped(stop)) return;  // the same for every thread of the block
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     sh[threadIdx.x] = c < n ? x[c] * x[c] : 0.0;
     __syncthreads();
@@ -261,6 +268,65 @@ __global__ __launch_bounds__(PB) void k_poly_sumsq(const double *x, uint32_t n, 
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Device-resident run, one block each.  flags: [0] stop, [1] converged,
+// [2] invalid dt, [3] steps completed.
+// The step's dt: minimum of the block partials (calc_timestep).
+__global__ __launch_bounds__(PB) void k_poly_step_dt(const double *part, uint32_t nb, int *flags, double *dtdev,
+                                                     double *dts, int k) {
+    __shared__ double sh[PB];
+    if (flags[0]) return;
+    double t = DBL_MAX;
+    for (uint32_t q = threadIdx.x; q < nb; q += PB) t = smin(t, part[q]);
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = PB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] = smin(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dt = sh[0];
+        dts[k] = dt;
+        if (!((dt - dt == 0.0) && dt > 0.0)) {
+            flags[2] = 1;
+            flags[0] = 1;
+        } else {
+            dtdev[0] = dt;
+        }
+    }
+}
+
+// The step's residual: sum of the block partials in a fixed order (each
+// thread a strided sum, then a tree), its square root, and the drop test
+// against ref (or this run's first residual when ref is 0).
+__global__ __launch_bounds__(PB) void k_poly_step_res(const double *part, uint32_t nb, int *flags, double *hres,
+                                                      int k, double drop, double ref) {
+    __shared__ double sh[PB];
+    if (flags[0]) return;
+    double v = 0.0;
+    for (uint32_t q = threadIdx.x; q < nb; q += PB) v += part[q];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = PB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double r = sqrt(sh[0]);
+        hres[k] = r;
+        flags[3] = k + 1;
+        const double r0 = ref > 0.0 ? ref : hres[0];
+        if (drop > 0.0 && r0 > 0.0 && r / r0 <= drop) {
+            flags[1] = 1;
+            flags[0] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(PB) void k_poly_copy(double *dst, const double *src, size_t n, const int *stop) {
+    const size_t q = (size_t)blockIdx.x * PB + threadIdx.x;
+    if (q < n && !stopped(stop)) dst[q] = src[q];
 }
 
 __global__ __launch_bounds__(PB) void k_poly_fill(double *p, double v, size_t n) {
@@ -279,6 +345,9 @@ struct PolyPath {
     double *grad = nullptr, *psi = nullptr, *R = nullptr, *part = nullptr;
     double *Rrho = nullptr;
     double *dtc = nullptr;  // [N] per-cell steps (local time stepping)
+    int *flags = nullptr;   // device-resident run flags
+    double *dtdev = nullptr, *dts = nullptr, *hres = nullptr;  // its dt, and per step dt and residual
+    int run_cap = 0;
     int *bad = nullptr;
     bool aux_valid = false;
     // run state since the last set_state / set_freestream
@@ -371,13 +440,17 @@ int tflush(PolyPath *P, std::string &err) {
 // primvars into V, ghosts, sweep A for second order, sweep B
 // lts_dt: also store each cell's CFL step of these primvars (first stage of a
 // locally time-stepped step); the update then uses the per-cell steps
-int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, bool lts_dt, std::string &err) {
-    const PolyDev &m = P->m;
+// stop, dtp: device-resident run (kernels return once *stop is set; the dt is
+// read from *dtp)
+int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res, bool lts_dt, std::string &err,
+                 const int *stop = nullptr, const double *dtp = nullptr) {
+    PolyDev m = P->m;
+    m.stop = stop;
     const bool check = P->prm.check_physical_validity != 0;
     int rc;
     hipEvent_t e = nullptr;
     if ((rc = tstart(P, &e, err))) return rc;
-    hipLaunchKernelGGL(k_poly_prim, grid(m.N), dim3(PB), 0, P->st, (const double *)P->U, V, m.N, m.NT);
+    hipLaunchKernelGGL(k_poly_prim, grid(m.N), dim3(PB), 0, P->st, (const double *)P->U, V, m.N, m.NT, stop);
     if ((rc = tstop(P, "poly_prim", e, err)) || (rc = tstart(P, &e, err))) return rc;
     if (m.G) hipLaunchKernelGGL(k_poly_ghost, grid(m.G), dim3(PB), 0, P->st, m, V, P->fs);
     if ((rc = tstop(P, "poly_ghost", e, err))) return rc;
@@ -404,6 +477,7 @@ int flux_balance(PolyPath *P, double *V, int kind, double dt, bool aux, bool res
     a.flux = P->prm.inv_flux_scheme == NF_FLUX_HLLC ? 1 : 0;
     a.kind = check ? -1 : kind;
     a.dt = dt;
+    a.dtp = dtp;
     a.dtc = lts ? P->dtc : nullptr;
     a.fs = P->fs;
     if (check) PHIP(hipMemsetAsync(P->bad, 0, sizeof(int), P->st));
@@ -576,7 +650,8 @@ int poly_set_state(PolyPath *P, const double *U, const double *U_old, std::strin
     int rc;
     reset_run(P);
     if ((rc = put_soa(P, P->U, U, 5, err))) return rc;
-    hipLaunchKernelGGL(k_poly_prim, grid(N), dim3(PB), 0, P->st, (const double *)P->U, P->V, N, P->m.NT);
+    hipLaunchKernelGGL(k_poly_prim, grid(N), dim3(PB), 0, P->st, (const double *)P->U, P->V, N, P->m.NT,
+                       (const int *)nullptr);
     if (zero)
         hipLaunchKernelGGL(k_poly_fill, grid(5 * (size_t)N), dim3(PB), 0, P->st, P->Uo, 0.0, 5 * (size_t)N);
     else
@@ -619,14 +694,18 @@ int poly_step(PolyPath *P, double dt, std::string &err) {
     PHIP(hipMemcpyAsync(P->Uo, P->U, 5 * (size_t)P->m.N * 8, hipMemcpyDeviceToDevice, P->st));
     // residual norm ||R_rho||_2 of the first stage
     const uint32_t nb = (P->m.N + PB - 1) / PB;
-    hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, P->m.N, P->part);
+    hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, P->m.N, P->part,
+                       (const int *)nullptr);
     PHIP(hipGetLastError());
     std::vector<double> part(nb);
     PHIP(hipMemcpyAsync(part.data(), P->part, nb * 8, hipMemcpyDeviceToHost, P->st));
     PHIP(hipStreamSynchronize(P->st));
-    double r2 = 0.0;
-    for (double x : part) r2 += x;
-    P->hist.push_back(std::sqrt(r2));
+    // the summation order of k_poly_step_res, so both loops give the same bits
+    std::vector<double> acc(PB, 0.0);
+    for (uint32_t q = 0; q < nb; ++q) acc[q % PB] += part[q];
+    for (int h = PB / 2; h > 0; h >>= 1)
+        for (int t = 0; t < h; ++t) acc[t] += acc[t + h];
+    P->hist.push_back(std::sqrt(acc[0]));
     P->time += dt;
     return NF_OK;
 }
@@ -646,8 +725,7 @@ int poly_get_state(PolyPath *P, double *U, std::string &err) { return get_soa(P,
 // The Driver loop (calc_timestep, step) for n steps, stopping after the step
 // whose residual norm is at most drop times the first step's; a run that has
 // reached its drop stays stopped until the state is set again.
-int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err) {
-    if (!o || o->n_steps < 0) return NF_E_ARG;
+static int poly_run_host(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err) {
     const double drop = o->residual_drop;
     hipEvent_t e0, e1;
     PHIP(hipEventCreate(&e0));
@@ -712,6 +790,102 @@ int poly_time_kernels(PolyPath *P, int32_t n_steps, nf_kernel_time *out, int32_t
     return NF_OK;
 }
 
+// One step of the device-resident run: the same kernels and values as
+// calc_timestep + step, with dt, residual and the stop test on the device.
+static int enqueue_run_step(PolyPath *P, int k, double drop, double ref, std::string &err) {
+    PolyDev m = P->m;
+    m.stop = P->flags;
+    const uint32_t nb = (m.N + PB - 1) / PB;
+    hipLaunchKernelGGL(k_poly_dt, dim3(nb), dim3(PB), 0, P->st, m, (const double *)P->V, P->prm.CFL, P->part,
+                       (double *)nullptr);
+    hipLaunchKernelGGL(k_poly_step_dt, dim3(1), dim3(PB), 0, P->st, (const double *)P->part, nb, P->flags,
+                       P->dtdev, P->dts, k);
+    PHIP(hipGetLastError());
+    const bool rk3 = P->prm.time_scheme == NF_TVD_RK3, lts = P->prm.local_time_step != 0;
+    int rc;
+    if ((rc = flux_balance(P, P->V, 0, 0.0, false, true, lts, err, P->flags, P->dtdev))) return rc;
+    if (rk3) {
+        if ((rc = flux_balance(P, P->V, 1, 0.0, false, false, false, err, P->flags, P->dtdev))) return rc;
+        if ((rc = flux_balance(P, P->V, 2, 0.0, false, false, false, err, P->flags, P->dtdev))) return rc;
+    }
+    hipLaunchKernelGGL(k_poly_copy, grid(5 * (size_t)m.N), dim3(PB), 0, P->st, P->Uo, (const double *)P->U,
+                       5 * (size_t)m.N, (const int *)P->flags);
+    hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, m.N, P->part,
+                       (const int *)P->flags);
+    hipLaunchKernelGGL(k_poly_step_res, dim3(1), dim3(PB), 0, P->st, (const double *)P->part, nb, P->flags,
+                       P->hres, k, drop, ref);
+    PHIP(hipGetLastError());
+    return NF_OK;
+}
+
+// nf_run: check_every steps are enqueued between host synchronisations; the
+// stop flag ends the remaining ones on the device, so the run stops after
+// exactly the step that reached the drop, as the host loop does.  With the
+// flux-balance check on (a host read per stage) the host loop runs instead.
+int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err) {
+    if (!o || o->n_steps < 0) return NF_E_ARG;
+    if (P->prm.check_physical_validity) return poly_run_host(P, o, stats, err);
+    const double drop = o->residual_drop;
+    int rc;
+    if (!P->flags && ((rc = alloc(P, &P->flags, 4, err)) || (rc = alloc(P, &P->dtdev, 1, err)))) return rc;
+    if (o->n_steps > P->run_cap) {  // per-step history buffers (the old ones stay owned until destroy)
+        const int cap = std::max(o->n_steps, 1024);
+        if ((rc = alloc(P, &P->dts, cap, err)) || (rc = alloc(P, &P->hres, cap, err))) return rc;
+        P->run_cap = cap;
+    }
+    PHIP(hipMemsetAsync(P->flags, 0, 4 * sizeof(int), P->st));
+    hipEvent_t e0, e1;
+    PHIP(hipEventCreate(&e0));
+    PHIP(hipEventCreate(&e1));
+    PHIP(hipEventRecord(e0, P->st));
+    const double ref = P->hist.empty() ? 0.0 : P->hist.front();
+    const int K = std::max(1, o->check_every);
+    int flags[4] = {0, 0, 0, 0}, queued = 0;
+    if (!(drop > 0.0 && P->converged)) {
+        while (queued < o->n_steps && !flags[0]) {
+            const int chunk = std::min(K, o->n_steps - queued);
+            for (int q = 0; q < chunk; ++q)
+                if ((rc = enqueue_run_step(P, queued + q, drop, ref, err))) return rc;
+            queued += chunk;
+            PHIP(hipMemcpyAsync(flags, P->flags, sizeof flags, hipMemcpyDeviceToHost, P->st));
+            PHIP(hipStreamSynchronize(P->st));
+        }
+    }
+    float ms = 0.0f;
+    (void)hipEventRecord(e1, P->st);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int done = flags[3];
+    std::vector<double> dts(done + 1), res(done);
+    if (done + flags[2] > 0) PHIP(hipMemcpy(dts.data(), P->dts, (done + flags[2]) * 8, hipMemcpyDeviceToHost));
+    if (done > 0) PHIP(hipMemcpy(res.data(), P->hres, done * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < done; ++k) {
+        P->time += dts[k];
+        P->hist.push_back(res[k]);
+    }
+    if (flags[1]) P->converged = true;
+    if (stats) {
+        stats->steps_done = done;
+        stats->converged = P->converged ? 1 : 0;
+        stats->time = P->time;
+        stats->last_dt = done > 0 ? dts[done - 1] : 0.0;
+        stats->res_first = P->hist.empty() ? 0.0 : P->hist.front();
+        stats->res_last = P->hist.empty() ? 0.0 : P->hist.back();
+        stats->elapsed_ms = ms;
+        stats->fell_back = 0;
+        stats->pad = 0;
+    }
+    if (flags[2]) {
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "Invalid dt calculated (dt = %f)", dts[done]);
+        err = buf;
+        return NF_E_INVALID_DT;
+    }
+    return NF_OK;
+}
+
 int poly_get_residuals(PolyPath *P, double *out, int32_t max, int32_t *n_out) {
     const int32_t n = (int32_t)std::min<size_t>(P->hist.size(), (size_t)std::max(max, 0));
     std::copy(P->hist.begin(), P->hist.begin() + n, out);
@@ -723,7 +897,7 @@ int poly_get_field(PolyPath *P, int32_t which, double *out, std::string &err) {
     switch (which) {
     case NF_FIELD_PRIM:  // prim(U) into Ve with row stride N for the export
         hipLaunchKernelGGL(k_poly_prim, grid(P->m.N), dim3(PB), 0, P->st, (const double *)P->U, P->Ve, P->m.N,
-                           P->m.N);
+                           P->m.N, (const int *)nullptr);
         PHIP(hipGetLastError());
         return get_soa(P, P->Ve, 5, out, err);
     case NF_FIELD_STATE_OLD:

@@ -176,6 +176,13 @@ def test_mixed_run_loop_and_residuals():
         t += cfg.delta_time
     np.testing.assert_array_equal(a.get_solver_data().get_solution(), b.get_solver_data().get_solution())
     assert st["time"] == t
+    # device-resident run and Driver loop: the same residual bits
+    np.testing.assert_array_equal(a.residuals(), b.residuals())
+    # with the flux-balance check the run loop synchronises per stage: same state
+    c = nf.EulerSolver(sod_config(check_physical_validity=True), mesh)
+    c.set_state(U0)
+    assert c.run(30, check_every=7)["steps_done"] == 30
+    np.testing.assert_array_equal(c.get_solver_data().get_solution(), b.get_solver_data().get_solution())
     ora.run(29)
     ora.eval_flux_balance()   # R(U^29), the 30th step's residual
     r = a.residuals()
