@@ -159,8 +159,8 @@ int nf_fv_export(const nf_mesh *mesh,
  * (src/Driver.cpp:31-57).  A mixed-element mesh (hexahedra, wedges,
  * pyramids) runs on the polyhedral FP64 path: one mesh, global or local time
  * steps, and the calls set_freestream, set_state, calc_timestep, step, run
- * (calc_timestep + step kept on the device, the host synchronising every
- * check_every steps; use_graph is ignored), get_residuals, time_kernels, eval_flux_balance, get_state,
+ * (calc_timestep + step kept on the device, check_every steps per HIP
+ * graph replay, the host synchronising between replays), get_residuals, time_kernels, eval_flux_balance, get_state,
  * get_field and ctx_sizes; the others return NF_E_ARG with a message, and
  * nf_create rejects NF_PREC_MIXED for such a mesh. */
 typedef struct nf_ctx nf_ctx;

@@ -271,12 +271,14 @@ __global__ __launch_bounds__(PB) void k_poly_sumsq(const double *x, uint32_t n, 
 }
 
 // Device-resident run, one block each.  flags: [0] stop, [1] converged,
-// [2] invalid dt, [3] steps completed.
+// [2] invalid dt, [3] steps completed (the index of the step in progress), so
+// a graph of several steps can be replayed unchanged.
 // The step's dt: minimum of the block partials (calc_timestep).
 __global__ __launch_bounds__(PB) void k_poly_step_dt(const double *part, uint32_t nb, int *flags, double *dtdev,
-                                                     double *dts, int k) {
+                                                     double *dts) {
     __shared__ double sh[PB];
     if (flags[0]) return;
+    const int k = flags[3];  // this step's index: the steps completed so far
     double t = DBL_MAX;
     for (uint32_t q = threadIdx.x; q < nb; q += PB) t = smin(t, part[q]);
     sh[threadIdx.x] = t;
@@ -301,9 +303,10 @@ __global__ __launch_bounds__(PB) void k_poly_step_dt(const double *part, uint32_
 // thread a strided sum, then a tree), its square root, and the drop test
 // against ref (or this run's first residual when ref is 0).
 __global__ __launch_bounds__(PB) void k_poly_step_res(const double *part, uint32_t nb, int *flags, double *hres,
-                                                      int k, double drop, double ref) {
+                                                      double drop, double ref) {
     __shared__ double sh[PB];
     if (flags[0]) return;
+    const int k = flags[3];
     double v = 0.0;
     for (uint32_t q = threadIdx.x; q < nb; q += PB) v += part[q];
     sh[threadIdx.x] = v;
@@ -792,14 +795,14 @@ int poly_time_kernels(PolyPath *P, int32_t n_steps, nf_kernel_time *out, int32_t
 
 // One step of the device-resident run: the same kernels and values as
 // calc_timestep + step, with dt, residual and the stop test on the device.
-static int enqueue_run_step(PolyPath *P, int k, double drop, double ref, std::string &err) {
+static int enqueue_run_step(PolyPath *P, double drop, double ref, std::string &err) {
     PolyDev m = P->m;
     m.stop = P->flags;
     const uint32_t nb = (m.N + PB - 1) / PB;
     hipLaunchKernelGGL(k_poly_dt, dim3(nb), dim3(PB), 0, P->st, m, (const double *)P->V, P->prm.CFL, P->part,
                        (double *)nullptr);
     hipLaunchKernelGGL(k_poly_step_dt, dim3(1), dim3(PB), 0, P->st, (const double *)P->part, nb, P->flags,
-                       P->dtdev, P->dts, k);
+                       P->dtdev, P->dts);
     PHIP(hipGetLastError());
     const bool rk3 = P->prm.time_scheme == NF_TVD_RK3, lts = P->prm.local_time_step != 0;
     int rc;
@@ -813,15 +816,16 @@ static int enqueue_run_step(PolyPath *P, int k, double drop, double ref, std::st
     hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, m.N, P->part,
                        (const int *)P->flags);
     hipLaunchKernelGGL(k_poly_step_res, dim3(1), dim3(PB), 0, P->st, (const double *)P->part, nb, P->flags,
-                       P->hres, k, drop, ref);
+                       P->hres, drop, ref);
     PHIP(hipGetLastError());
     return NF_OK;
 }
 
-// nf_run: check_every steps are enqueued between host synchronisations; the
-// stop flag ends the remaining ones on the device, so the run stops after
-// exactly the step that reached the drop, as the host loop does.  With the
-// flux-balance check on (a host read per stage) the host loop runs instead.
+// nf_run: check_every steps, captured once as a HIP graph (use_graph) or
+// launched eagerly, run between host synchronisations; the stop flag ends the
+// remaining ones on the device, so the run stops after exactly the step that
+// reached the drop, as the host loop does.  With the flux-balance check on (a
+// host read per stage) the host loop runs instead.
 int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string &err) {
     if (!o || o->n_steps < 0) return NF_E_ARG;
     if (P->prm.check_physical_validity) return poly_run_host(P, o, stats, err);
@@ -841,11 +845,35 @@ int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string
     const double ref = P->hist.empty() ? 0.0 : P->hist.front();
     const int K = std::max(1, o->check_every);
     int flags[4] = {0, 0, 0, 0}, queued = 0;
+    struct GraphExec {
+        hipGraphExec_t g = nullptr;
+        ~GraphExec() {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } ge;
+    hipGraphExec_t &graph = ge.g;
     if (!(drop > 0.0 && P->converged)) {
         while (queued < o->n_steps && !flags[0]) {
             const int chunk = std::min(K, o->n_steps - queued);
-            for (int q = 0; q < chunk; ++q)
-                if ((rc = enqueue_run_step(P, queued + q, drop, ref, err))) return rc;
+            if (o->use_graph && chunk == K && K > 1) {
+                if (!graph) {
+                    hipGraph_t gr;
+                    PHIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+                    for (int q = 0; q < K; ++q)
+                        if ((rc = enqueue_run_step(P, drop, ref, err))) {
+                            (void)hipStreamEndCapture(P->st, &gr);
+                            return rc;
+                        }
+                    PHIP(hipStreamEndCapture(P->st, &gr));
+                    const hipError_t ie = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(gr);
+                    PHIP(ie);
+                }
+                PHIP(hipGraphLaunch(graph, P->st));
+            } else {
+                for (int q = 0; q < chunk; ++q)
+                    if ((rc = enqueue_run_step(P, drop, ref, err))) return rc;
+            }
             queued += chunk;
             PHIP(hipMemcpyAsync(flags, P->flags, sizeof flags, hipMemcpyDeviceToHost, P->st));
             PHIP(hipStreamSynchronize(P->st));
