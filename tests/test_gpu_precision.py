@@ -1,0 +1,171 @@
+"""Mixed precision (FP32 gradient, limiter, reconstruction and flux; FP64
+state, flux-balance sums, RK combination, time step and residual norms)
+against the FP64 oracle.
+
+The reference's only precision knob is the all-float `Scalar` switch
+(include/Includes.hpp:39-45); its DBL_MAX initialisers
+(include/Numerics.hpp:179, 267-268) become FLT_MAX / FLT_EPSILON in the FP32
+limiter.  Each test states its error bound: FP32 face arithmetic (unit
+round-off 6e-8) accumulated over the stages run.  Per-variable errors are
+measured relative to the norm of the variable's own field, and the momentum
+components relative to the norm of the whole momentum field (a component
+that is zero in the exact solution has no scale of its own)."""
+import math
+
+import numpy as np
+import pytest
+
+import netflux_amd as nf
+from tests.cases import pulse_config, pulse_state, rel_l2, sod_config, sod_state
+from tests.oracle_bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(mesh, cfg, U0, steps, uold_equals_u=True, block=None, check_every=10):
+    print(f"[case] precision={cfg.precision} cells={mesh.n_cells} steps={steps} block={block}", flush=True)
+    cfg.rk3_zero_old_first_step = not uold_equals_u
+    g = nf.EulerSolver(cfg, mesh)
+    if block is not None:
+        g.set_launch(block, True)
+    g.set_state(U0, U0 if uold_equals_u else None)
+    st = g.run(steps, use_graph=True, check_every=check_every)
+    return g, st
+
+
+def _oracle(mesh, cfg, U0, steps, uold_equals_u=True):
+    cfg.rk3_zero_old_first_step = not uold_equals_u
+    o = Oracle.from_mesh(mesh, cfg)
+    o.set_state(U0, uold_equals_u=uold_equals_u)
+    o.run(steps)
+    return o
+
+
+def per_variable_errors(Ug, Uo):
+    """Relative L2 error of each conserved variable (momentum components
+    against the norm of the momentum field)."""
+    mom = np.linalg.norm(Uo[:, 1:4])
+    scale = [np.linalg.norm(Uo[:, 0]), mom, mom, mom, np.linalg.norm(Uo[:, 4])]
+    return [float(np.linalg.norm(Ug[:, k] - Uo[:, k]) / scale[k]) for k in range(5)]
+
+
+def test_mixed_sod_error_vs_fp64_oracle():
+    """16^3 x 6 tets, Sod, Barth, 50 TVD-RK3 steps (150 FP32 stages)."""
+    mesh = nf.Mesh.box(16)
+    U0 = sod_state(mesh)
+    g, st = _run(mesh, sod_config(precision="mixed"), U0, 50)
+    assert st["steps_done"] == 50
+    o = _oracle(mesh, sod_config(), U0, 50)
+    Ug, Uo = g.get_solver_data().get_solution(), o.get("U")
+    errs = per_variable_errors(Ug, Uo)
+    print("mixed vs FP64 oracle, per variable:", ["%.2e" % e for e in errs], "total %.2e" % rel_l2(Ug, Uo))
+    assert rel_l2(Ug, Uo) < 2e-5
+    assert max(errs) < 2e-5
+    # and it is not FP64: the FP32 arithmetic is really in use
+    assert rel_l2(Ug, Uo) > 1e-12
+
+
+def test_mixed_flux_balance_vs_fp64_oracle():
+    """One evaluation of R, the gradient and the limiter in FP32 against the
+    FP64 oracle: errors of the order of the FP32 round-off."""
+    mesh = nf.Mesh.box(8, jitter=0.12, seed=7)
+    U0 = sod_state(mesh)
+    g = nf.EulerSolver(sod_config(precision="mixed"), mesh)
+    g.set_state(U0)
+    g.eval_flux_balance()
+    o = Oracle.from_mesh(mesh, sod_config())
+    o.set_state(U0)
+    o.eval_flux_balance()
+    d = g.get_solver_data()
+    eR = rel_l2(d.get_flux_balance(), o.get("R"))
+    eG = rel_l2(d.get_primvars_gradient(), o.get("grad"))
+    print(f"R {eR:.2e}, gradient {eG:.2e}")
+    assert eR < 1e-5 and eG < 1e-5
+    # limiter: FP32 quotients near a kink of min(1, .) can move by more than
+    # the round-off; bound the share of cells that differ noticeably
+    dpsi = np.abs(d.get_primvars_limiter() - o.get("psi"))
+    assert np.mean(dpsi > 1e-4) < 0.01
+
+
+def test_mixed_conserves_mass_and_energy():
+    mesh = nf.Mesh.box(10, jitter=0.15, seed=12)
+    U0 = sod_state(mesh)
+    vol = mesh.fv_grid()["volume"][: mesh.n_cells]
+    drift = {}
+    for prec in ("fp64", "mixed"):
+        g, _ = _run(mesh, sod_config(precision=prec), U0, 40)
+        U = g.get_solver_data().get_solution()
+        drift[prec] = [abs(vol @ U[:, k] - vol @ U0[:, k]) / (vol @ U0[:, k]) for k in (0, 4)]
+    print("relative drift of mass and energy after 40 steps:", drift)
+    # Interior faces cancel exactly in both modes (each face flux is evaluated
+    # bit-identically by its two cells and summed in FP64).  The slip walls
+    # leak at the rounding level of the reflected ghost state: FP64 there,
+    # FP32 in mixed mode (the unit normal S/|S| is an FP32 quantity).
+    assert max(drift["fp64"]) < 1e-13
+    assert max(drift["mixed"]) < 1e-9
+
+
+def test_mixed_interior_fluxes_cancel_exactly():
+    """At rest (u = 0) next to slip walls the wall mass and energy fluxes are
+    exactly zero, so the cell flux balances sum to exactly zero iff each
+    interior face flux enters its two cells with opposite signs bit for bit.
+    math.fsum of the FP64 balances is exact."""
+    mesh = nf.Mesh.box(10, jitter=0.15, seed=12)
+    g = nf.EulerSolver(sod_config(precision="mixed"), mesh)
+    g.set_state(sod_state(mesh))
+    g.eval_flux_balance()
+    R = g.get_solver_data().get_flux_balance()
+    assert np.abs(R[:, 0]).max() > 0
+    assert math.fsum(R[:, 0]) == 0.0 and math.fsum(R[:, 4]) == 0.0
+
+
+def test_mixed_farfield_pulse():
+    mesh = nf.Mesh.box(12, jitter=0.15, seed=5, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    g, _ = _run(mesh, pulse_config(precision="mixed"), U0, 30)
+    o = _oracle(mesh, pulse_config(), U0, 30)
+    Ug, Uo = g.get_solver_data().get_solution(), o.get("U")
+    errs = per_variable_errors(Ug, Uo)
+    print("farfield pulse, per variable:", ["%.2e" % e for e in errs])
+    assert rel_l2(Ug, Uo) < 1e-5 and max(errs) < 1e-5
+
+
+def test_precision_switch_continues_in_fp64():
+    """Mixed for 10 steps, then FP64 for 10 in the same context, equals a FP64
+    context started from the mixed state (primvars and dt recomputed from U)."""
+    mesh = nf.Mesh.box(8, jitter=0.1, seed=21, patches=("FarField",) * 6)
+    U0 = pulse_state(mesh)
+    a, _ = _run(mesh, pulse_config(precision="mixed"), U0, 10)
+    U10 = a.get_solver_data().get_solution()
+    a.set_precision("fp64")
+    a.run(10, use_graph=True, check_every=5)
+    b, _ = _run(mesh, pulse_config(precision="fp64"), U10, 10)
+    np.testing.assert_array_equal(a.get_solver_data().get_solution(), b.get_solver_data().get_solution())
+
+
+@pytest.mark.parametrize("block", [256, 128])
+def test_mixed_bump_channel_barth(block):
+    """A small channel of the mixed-precision workload's shape: [0,3]x[0,1]x
+    [0,0.5] with the Gaussian floor bump (h 0.05, sigma 0.2), farfield x-planes
+    at Mach 0.5, slip walls elsewhere, Barth in FP32, 60 RK3 steps from the
+    freestream with U_old = U^0 (the steady-run setting).  Block 128 and 256
+    launches must give the same state bit for bit (the LDS neighbour copy only
+    changes where the same values are read from); both are held to the FP64
+    oracle."""
+    mesh = nf.Mesh.bump_channel(30, 10, 5, h=0.05)
+    cfg = pulse_config(mach=0.5, precision="mixed")
+    V = np.tile([1.0, 0.5, 0.0, 0.0, 1 / 1.4], (mesh.n_cells, 1))
+    from tests.cases import prim_to_cons
+    U0 = prim_to_cons(V)
+    g, st = _run(mesh, cfg, U0, 60, block=block)
+    Ug = g.get_solver_data().get_solution()
+    o = _oracle(mesh, pulse_config(mach=0.5), U0, 60)
+    Uo = o.get("U")
+    errs = per_variable_errors(Ug, Uo)
+    # the flow must have moved away from the uniform start
+    assert np.linalg.norm(Uo - U0) > 1e-3 * np.linalg.norm(U0)
+    print(f"bump channel block {block}, per variable:", ["%.2e" % e for e in errs])
+    assert rel_l2(Ug, Uo) < 1e-5 and max(errs) < 1e-4
+    other = 128 if block == 256 else 256
+    h, _ = _run(mesh, pulse_config(mach=0.5, precision="mixed"), U0, 60, block=other)
+    np.testing.assert_array_equal(h.get_solver_data().get_solution(), Ug)

@@ -3,8 +3,10 @@ what mathematics fixes, independently of its own formulas:
 
   physics   textbook Euler flux, cons<->prim round trip
   Rusanov   consistency F(U,U,S) = |S| F(U,n), exact antisymmetry, 1-D closed form
-  BCs       slip-wall reflection, farfield super-/subsonic limits (the
-            reference's own asserts, src/Numerics.cpp:159-170)
+  BCs       slip-wall reflection, farfield supersonic limits (the
+            reference's own asserts, src/Numerics.cpp:159-170), farfield
+            subsonic in- and outflow against the Riemann-invariant relations
+            (with the reference's sign and density quirks)
   gradient  zero for constant fields, telescoping identity, linear fields on a
             uniform mesh
   limiter   psi in [0,1], reconstructed face values inside the neighbour
@@ -114,6 +116,107 @@ def test_farfield_limits():
         np.testing.assert_allclose(ob.ghost(2, Vfs, Vfs, S), inverted(Vfs), rtol=1e-12, atol=1e-14)
         Vfs[0] = 1.0
         np.testing.assert_allclose(ob.ghost(2, Vfs, Vfs, S), Vfs, rtol=1e-12, atol=1e-14)
+
+
+def _isentropic_density(s, c):
+    """Density with entropy s = p rho^-gamma and sound speed c (c^2 = gamma p / rho)."""
+    return (c * c / (G * s)) ** (1 / (G - 1))
+
+
+def farfield_characteristic_violations(Vfs, Vd, Vg, S):
+    """Checks a farfield ghost state Vg of the domain state Vd, freestream
+    Vfs and outward face vector S for a subsonic freestream normal velocity
+    (|v_n,fs| < c_fs), from the theory of Riemann invariants rather than from
+    the reference's expressions:
+      - the outgoing invariant R+ = v_n + 2c/(gamma-1) comes from the domain,
+        the incoming R- = v_n - 2c/(gamma-1) from the freestream, so the
+        boundary state has v_n,b = (R+ + R-)/2 and c_b = (gamma-1)/4 (R+ - R-);
+        the ghost's own sound speed must be c_b;
+      - with v_n,fs > 0 (outflow) the tangential velocity and the entropy
+        come from the domain and the ghost's normal velocity is v_n,b; with
+        v_n,fs <= 0 (inflow) they come from the freestream and the ghost's
+        normal velocity is 2 v_n,fs - v_n,b (src/Numerics.cpp:143-147: the
+        freestream velocity plus (v_n,fs - v_n,b) n);
+      - the ghost density is the reciprocal of the isentropic density at
+        (entropy of the source side, c_b): the reference's
+        (gamma s / c_b^2)^(1/(gamma-1)) inverts the textbook relation.
+    Returns the list of violated properties (empty if all hold)."""
+    n = S / np.linalg.norm(S)
+    c = lambda V: np.sqrt(G * V[4] / V[0])  # noqa: E731
+    s = lambda V: V[4] * V[0] ** -G  # noqa: E731
+    vfs, vd, vg = Vfs[1:4], Vd[1:4], Vg[1:4]
+    vn_fs, vn_d, vn_g = vfs @ n, vd @ n, vg @ n
+    assert abs(vn_fs) < c(Vfs)
+    Rp = vn_d + 2 * c(Vd) / (G - 1)
+    Rm = vn_fs - 2 * c(Vfs) / (G - 1)
+    vn_b, c_b = 0.5 * (Rp + Rm), (G - 1) / 4 * (Rp - Rm)
+    out = []
+    if not np.isclose(c(Vg), c_b, rtol=1e-12, atol=0):
+        out.append("sound speed")
+    src = Vd if vn_fs > 0 else Vfs
+    tang = lambda v: v - (v @ n) * n  # noqa: E731
+    if not np.allclose(tang(vg), tang(src[1:4]), rtol=1e-12, atol=1e-14):
+        out.append("tangential velocity")
+    want_vn = vn_b if vn_fs > 0 else 2 * vn_fs - vn_b
+    if not np.isclose(vn_g, want_vn, rtol=1e-12, atol=1e-14):
+        out.append("normal velocity")
+    if not np.isclose(Vg[0] * _isentropic_density(s(src), c_b), 1.0, rtol=1e-12, atol=0):
+        out.append("density")
+    return out
+
+
+def _farfield_variant(Vfs, Vd, S, swap_side=False, flip_q5=False, rminus_from_domain=False,
+                      rplus_from_fs=False):
+    """A numpy restatement of BC_FarField::calc_ghost_val (src/Numerics.cpp:
+    92-156) with optional planted mistakes; only used to show that the checks
+    above reject each of them."""
+    n = S / np.linalg.norm(S)
+    c = lambda V: np.sqrt(G * V[4] / V[0])  # noqa: E731
+    s = lambda V: V[4] * V[0] ** -G  # noqa: E731
+    vfs, vd = Vfs[1:4], Vd[1:4]
+    vn_fs, vn_d = vfs @ n, vd @ n
+    Rp = (vn_fs + 2 * c(Vfs) / (G - 1)) if (rplus_from_fs or vn_fs + c(Vfs) <= 0) else vn_d + 2 * c(Vd) / (G - 1)
+    Rm = (vn_d - 2 * c(Vd) / (G - 1)) if (rminus_from_domain or vn_fs - c(Vfs) > 0) else vn_fs - 2 * c(Vfs) / (G - 1)
+    vn_b, c_b = 0.5 * (Rp + Rm), (G - 1) / 4 * (Rp - Rm)
+    if (vn_fs > 0) != swap_side:
+        vel, ent = vd + (vn_b - vn_d) * n, s(Vd)
+    else:
+        vel = vfs + ((vn_b - vn_fs) if flip_q5 else (vn_fs - vn_b)) * n
+        ent = s(Vfs)
+    rho = (G * ent / (c_b * c_b)) ** (1 / (G - 1))
+    return np.array([rho, *vel, c_b * c_b * rho / G])
+
+
+def test_farfield_subsonic_branches():
+    """Subsonic in- and outflow with a domain state different from the
+    freestream (densities != 1, oblique face normals): the oracle's ghost
+    satisfies the characteristic relations above, and each plausible mistake
+    (tangential side swapped, the Q5 sign 'fixed', R- taken from the domain,
+    R+ taken from the freestream) violates at least one of them."""
+    r = np.random.default_rng(77)
+    n_in = n_out = 0
+    for _ in range(200):
+        S = r.normal(size=3) * r.uniform(0.1, 3.0)
+        n = S / np.linalg.norm(S)
+        rho_fs, p_fs = r.uniform(0.5, 2.0), r.uniform(0.5, 2.0)
+        c_fs = np.sqrt(G * p_fs / rho_fs)
+        # freestream velocity with a subsonic normal component of either sign
+        vt = r.normal(size=3)
+        vt -= (vt @ n) * n
+        vn = r.uniform(-0.95, 0.95) * c_fs
+        Vfs = np.array([rho_fs, *(vt + vn * n), p_fs])
+        Vd = np.array([r.uniform(0.5, 2.0), *r.normal(scale=0.5, size=3), r.uniform(0.5, 2.0)])
+        Vg = ob.ghost(2, Vfs, Vd, S)
+        assert farfield_characteristic_violations(Vfs, Vd, Vg, S) == []
+        np.testing.assert_allclose(_farfield_variant(Vfs, Vd, S), Vg, rtol=1e-12, atol=1e-14)
+        for mistake in ("swap_side", "flip_q5", "rminus_from_domain", "rplus_from_fs"):
+            if mistake == "flip_q5" and vn > 0:
+                continue  # the Q5 term is on the inflow side only
+            bad = _farfield_variant(Vfs, Vd, S, **{mistake: True})
+            assert farfield_characteristic_violations(Vfs, Vd, bad, S), mistake
+        n_in += vn <= 0
+        n_out += vn > 0
+    assert n_in > 50 and n_out > 50
 
 
 def _farfield_mesh(n=5, jitter=0.12):
