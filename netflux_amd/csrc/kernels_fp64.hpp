@@ -442,9 +442,12 @@ __device__ __forceinline__ void face_geo(const DevMesh &m, const CellGeo &g, con
     }
 }
 
-// wave + block min / sum over all threads; result valid in thread 0
+// wave + block min / sum over all threads; result valid in thread 0.  The min
+// skips NaN inputs, as the reference's running std::min(delta_time, x) does
+// (x = NaN compares false and leaves delta_time), whatever lane holds them.
 template <bool MIN, int B>
 __device__ __forceinline__ double block_reduce(double v, double *sh) {
+    if (MIN && !(v == v)) v = DBL_MAX;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const double w = __shfl_xor(v, o, 64);
@@ -477,6 +480,77 @@ __device__ __forceinline__ double cell_dt(const double *Vc, const V3d *S, const 
     const double sy = (fabs(Vc[2]) + cs) * d1;
     const double sz = (fabs(Vc[3]) + cs) * d2;
     return CFL * vol / ((sx + sy) + sz);
+}
+
+// Stage combination of Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:
+// 98-132) with f = (stage weight x dt) / volume: kind 1 U + f R, kind 2
+// 3/4 U_old + 1/4 U + f R, kind 3 1/3 U_old + 2/3 U + f R (U_old = 0 on the
+// first step of a zero-U_old run).  Shared by the fused sweeps and the checked
+// stage update, so both give the same bits in a translation unit.
+__device__ __forceinline__ void stage_combine(int kind, bool zero_old, double f, const double *Up,
+                                              const double *Uop, const double *R, double *Un) {
+    if (kind == 1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Un[k] = Up[k] + f * R[k];
+    } else if (kind == 2) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double uo = zero_old ? 0.0 : Uop[k];
+            Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * Up[k] + f * R[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double uo = zero_old ? 0.0 : Uop[k];
+            Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * Up[k] + f * R[k];
+        }
+    }
+}
+
+// Checked stage (check_physical_validity): the sweep stores R [tile][5][64]
+// and f [cell] instead of updating; this pass updates only if no cell of the
+// instance had a non-finite flux balance.  Otherwise the instance stops with
+// U untouched and bad_flux holding that one evaluation's count, as the
+// reference throws at the end of evaluate_flux_balance before any update
+// (src/Solver.cpp:80, src/SolverData.cpp:66-79).  One lane per device cell,
+// BLOCK-cell blocks (the instance padding unit).
+struct ApplyArgs {
+    const double *U, *Uo;
+    double *Uout;
+    void *Vout;  // prim(Uout): double (FP64 path, may be null) or float (mixed path)
+    const double *Rst, *fst;
+    int kind;
+};
+
+template <typename VT>
+__device__ __forceinline__ void apply_stage_cell(const DevMesh &m, const ApplyArgs &a) {
+    const uint32_t inst = m.blk_inst[blockIdx.x];
+    InstState &st = m.st[inst];
+    if (st.stop) return;
+    if (st.bad_flux > 0) {
+        if (threadIdx.x == 0) st.stop = 1;
+        return;
+    }
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= m.inst_end[inst]) return;
+    const bool zero_old = st.uold_zero;
+    const double *uo = (a.kind == 1 || zero_old) ? a.U : a.Uo;
+    double Up[5], Uop[5], R[5], Un[5], Vnew[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        Up[k] = a.U[tix(c, k)];
+        Uop[k] = uo[tix(c, k)];
+        R[k] = a.Rst[tix(c, k)];
+    }
+    stage_combine(a.kind, zero_old, a.fst[c], Up, Uop, R, Un);
+    cons_to_prim_d(Un, Vnew);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a.Uout[tix(c, k)] = Un[k];
+    if (a.Vout) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) static_cast<VT *>(a.Vout)[tix(c, k)] = (VT)Vnew[k];
+    }
+    if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
 }
 
 #ifndef NF_DEVICE_HELPERS_ONLY
@@ -764,6 +838,7 @@ struct SweepBArgs {
     int check;
     int xcd_remap;
     double *R_aux;          // [N][5] or null: store R, skip the update
+    double *Rst, *fst;      // checked stage: R [tile][5][64] and f [cell] for k_apply, no update
 };
 
 // FLUX: 0 Rusanov, 1 HLLC
@@ -855,36 +930,27 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         }
     } else {
         const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
-        double Un[5], Vnew[5];
         // Solver::TVD_RK3 / explicit_euler (src/Solver.cpp:98-132)
-        if (a.kind == 1) {
-            const double f = rdiv(dt, g.ivol);
+        const double f = rdiv(a.kind == 1 ? dt : (a.kind == 2 ? 1.0 / 4.0 * dt : 2.0 / 3.0 * dt), g.ivol);
+        if (a.Rst) {
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Un[k] = Up[k] + f * R[k];
-        } else if (a.kind == 2) {
-            const double f = rdiv(1.0 / 4.0 * dt, g.ivol);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : Uop[k];
-                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * Up[k] + f * R[k];
+                for (int k = 0; k < 5; ++k) a.Rst[tix(c, k)] = R[k];
+                a.fst[c] = f;
             }
         } else {
-            const double f = rdiv(2.0 / 3.0 * dt, g.ivol);
+            double Un[5], Vnew[5];
+            stage_combine(a.kind, zero_old, f, Up, Uop, R, Un);
+            cons_to_prim_d(Un, Vnew);
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : Uop[k];
-                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * Up[k] + f * R[k];
+                for (int k = 0; k < 5; ++k) a.Uout[tix(c, k)] = Un[k];
+                if (a.Vout) {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) a.Vout[tix(c, k)] = Vnew[k];
+                }
+                if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
             }
-        }
-        cons_to_prim_d(Un, Vnew);
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) a.Uout[tix(c, k)] = Un[k];
-            if (a.Vout) {
-#pragma unroll
-                for (int k = 0; k < 5; ++k) a.Vout[tix(c, k)] = Vnew[k];
-            }
-            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
         }
     }
     if (a.res_part) {
@@ -893,6 +959,8 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         if (threadIdx.x == 0) a.res_part[L.lb] = v;
     }
 }
+
+__global__ __launch_bounds__(BLOCK) void k_apply(DevMesh m, ApplyArgs a) { apply_stage_cell<double>(m, a); }
 
 // prim of a tiled state: V = prim(U)
 __global__ __launch_bounds__(BLOCK) void k_prim(const double *U, double *V, uint32_t N) {

@@ -50,6 +50,7 @@ struct SweepBArgsMx {
     int check;
     int xcd_remap;
     double *R_aux;
+    double *Rst, *fst;  // checked stage: R and f stored for the apply pass, no update
 };
 
 #ifdef NF_MIXED_KERNELS
@@ -341,34 +342,25 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
         }
     } else {
         const double dt = a.dt_cell ? a.dt_cell[c] : st.dt;
-        double Un[5], Vnew[5];
-        if (a.kind == 1) {
-            const double f = dt / g.vol;
+        const double f = (a.kind == 1 ? dt : (a.kind == 2 ? 1.0 / 4.0 * dt : 2.0 / 3.0 * dt)) / g.vol;
+        if (a.Rst) {
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) Un[k] = Up[k] + f * R[k];
-        } else if (a.kind == 2) {
-            const double f = 1.0 / 4.0 * dt / g.vol;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : Uop[k];
-                Un[k] = 3.0 / 4.0 * uo + 1.0 / 4.0 * Up[k] + f * R[k];
+                for (int k = 0; k < 5; ++k) a.Rst[tix(c, k)] = R[k];
+                a.fst[c] = f;
             }
         } else {
-            const double f = 2.0 / 3.0 * dt / g.vol;
+            double Un[5], Vnew[5];
+            stage_combine(a.kind, zero_old, f, Up, Uop, R, Un);
+            cons_to_prim_d(Un, Vnew);
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const double uo = zero_old ? 0.0 : Uop[k];
-                Un[k] = 1.0 / 3.0 * uo + 2.0 / 3.0 * Up[k] + f * R[k];
+                for (int k = 0; k < 5; ++k) {
+                    a.Uout[tix(c, k)] = Un[k];
+                    a.Vout[tix(c, k)] = (float)Vnew[k];
+                }
+                if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
             }
-        }
-        cons_to_prim_d(Un, Vnew);
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                a.Uout[tix(c, k)] = Un[k];
-                a.Vout[tix(c, k)] = (float)Vnew[k];
-            }
-            if (!(Vnew[0] > 0.0) || !(Vnew[4] > 0.0) || !(Un[4] > 0.0)) atomicAdd(&st.bad_state, 1);
         }
     }
     if (a.res_part) {
@@ -377,6 +369,8 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
         if (threadIdx.x == 0) a.res_part[L.lb] = v;
     }
 }
+
+__global__ __launch_bounds__(BLOCK) void k_apply_mx(DevMesh m, ApplyArgs a) { apply_stage_cell<float>(m, a); }
 
 __global__ __launch_bounds__(BLOCK) void k_prim_mx(const double *U, float *V, uint32_t N) {
     const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
@@ -396,6 +390,7 @@ hipError_t launch_ghost_mx(bool face, dim3 g, hipStream_t st, const DevMesh &m, 
 hipError_t launch_sweepA_mx(int order, int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepAArgsMx &a);
 hipError_t launch_sweepB_mx(int block, dim3 g, hipStream_t st, const DevMesh &m, const SweepBArgsMx &a);
 hipError_t launch_prim_mx(dim3 g, hipStream_t st, const double *U, float *V, uint32_t N);
+hipError_t launch_apply_mx(dim3 g, hipStream_t st, const DevMesh &m, const ApplyArgs &a);
 
 }  // namespace mx
 }  // namespace nf

@@ -237,6 +237,7 @@ struct nf_ctx {
     FarField<float> *fs32 = nullptr;
     double *dt_cell = nullptr;
     double *grad_aux = nullptr, *psi_aux = nullptr, *R_aux = nullptr;
+    double *Rst = nullptr, *fst = nullptr;           // checked stages: R and f for k_apply
     double *dt_part = nullptr, *res_part = nullptr;  // per sweep block
     RedTable red256, red128;
     double *red_dt = nullptr, *red_res = nullptr;  // per chunk
@@ -400,32 +401,45 @@ int run_sweepA(nf_ctx *ctx, bool dt, bool aux, bool dtc = false, const double *U
 }
 
 // sweep B: stage `kind` of the update from U into Uout, prim(Uout) into V.
-// res: per-block sums of R_rho^2; aux: keep R.
+// res: per-block sums of R_rho^2; aux: keep R.  With check_physical_validity
+// the sweep stores R and the stage factor, and k_apply updates only the
+// instances whose flux balance is finite everywhere (the others stop with U
+// untouched, as the reference throws before the update).
 int run_sweepB(nf_ctx *ctx, int kind, const double *U, double *Uout, bool res, bool aux) {
     int rc;
     if ((rc = run_ghost(ctx, true))) return rc;
     const DevMesh m = devmesh(ctx);
     const double *dtc = ctx->prm.local_time_step ? ctx->dt_cell : nullptr;
+    const bool checked = ctx->prm.check_physical_validity && !aux;
+    double *Rst = checked ? ctx->Rst : nullptr, *fst = checked ? ctx->fst : nullptr;
+    const ApplyArgs ap{U, ctx->Ua, Uout, ctx->mixed ? (void *)ctx->V32 : nullptr, Rst, fst, kind};
     if (ctx->mixed) {
         const mx::SweepBArgsMx x{ctx->FV32, ctx->Gf32, U, ctx->Ua, Uout, ctx->V32, dtc, kind,
                                  res ? ctx->res_part : nullptr, ctx->prm.check_physical_validity,
-                                 ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
-        return launch_fn(ctx, "sweepB_mx", [&] {
-            return mx::launch_sweepB_mx(ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
-        });
+                                 ctx->xcd_remap, aux ? ctx->R_aux : nullptr, Rst, fst};
+        if ((rc = launch_fn(ctx, "sweepB_mx", [&] {
+                 return mx::launch_sweepB_mx(ctx->block, dim3(ctx->Ncell / ctx->block), ctx->stream, m, x);
+             })))
+            return rc;
+        if (!checked) return NF_OK;
+        return launch_fn(ctx, "apply_mx",
+                         [&] { return mx::launch_apply_mx(dim3(ctx->Ncell / BLOCK), ctx->stream, m, ap); });
     }
     // (FP64: prim(Uout) is not stored; the next sweep A evaluates it from U)
     const SweepBArgs a{ctx->FV, ctx->Gf, U, ctx->Ua, Uout, nullptr, dtc, kind, res ? ctx->res_part : nullptr,
-                       ctx->prm.check_physical_validity, ctx->xcd_remap, aux ? ctx->R_aux : nullptr};
+                       ctx->prm.check_physical_validity, ctx->xcd_remap, aux ? ctx->R_aux : nullptr, Rst, fst};
     const bool hllc = ctx->prm.inv_flux_scheme == NF_FLUX_HLLC;
-    return launch_fn(ctx, "sweepB", [&] {
-        const dim3 g(ctx->Ncell / ctx->block), b(ctx->block);
-        if (ctx->block == 128 && hllc) hipLaunchKernelGGL((k_sweepB<128, 1>), g, b, 0, ctx->stream, m, a);
-        else if (ctx->block == 128) hipLaunchKernelGGL((k_sweepB<128, 0>), g, b, 0, ctx->stream, m, a);
-        else if (hllc) hipLaunchKernelGGL((k_sweepB<256, 1>), g, b, 0, ctx->stream, m, a);
-        else hipLaunchKernelGGL((k_sweepB<256, 0>), g, b, 0, ctx->stream, m, a);
-        return hipGetLastError();
-    });
+    if ((rc = launch_fn(ctx, "sweepB", [&] {
+             const dim3 g(ctx->Ncell / ctx->block), b(ctx->block);
+             if (ctx->block == 128 && hllc) hipLaunchKernelGGL((k_sweepB<128, 1>), g, b, 0, ctx->stream, m, a);
+             else if (ctx->block == 128) hipLaunchKernelGGL((k_sweepB<128, 0>), g, b, 0, ctx->stream, m, a);
+             else if (hllc) hipLaunchKernelGGL((k_sweepB<256, 1>), g, b, 0, ctx->stream, m, a);
+             else hipLaunchKernelGGL((k_sweepB<256, 0>), g, b, 0, ctx->stream, m, a);
+             return hipGetLastError();
+         })))
+        return rc;
+    if (!checked) return NF_OK;
+    return LAUNCH("apply", k_apply, dim3(ctx->Ncell / BLOCK), dim3(BLOCK), m, ap);
 }
 
 int reduce_dt(nf_ctx *ctx, int advance) {
@@ -928,6 +942,10 @@ int create(int32_t n, const nf_mesh_desc *descs, const nf_params *prms, nf_ctx *
         if ((rc = dalloc(ctx, &ctx->Gf, 5 * ((size_t)ctx->nbf + TILE - 1) / TILE * TILE))) return rc;
     }
     if (p0.local_time_step && (rc = dalloc(ctx, &ctx->dt_cell, NC))) return rc;
+    if (p0.check_physical_validity) {
+        if ((rc = dalloc(ctx, &ctx->Rst, 5 * NC))) return rc;
+        if ((rc = dalloc(ctx, &ctx->fst, NC))) return rc;
+    }
     if ((rc = dalloc(ctx, &ctx->bface, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->bbc, ctx->nbf))) return rc;
     if ((rc = dalloc(ctx, &ctx->bS, 3 * (size_t)ctx->nbf))) return rc;

@@ -38,5 +38,10 @@ hipError_t launch_prim_mx(dim3 g, hipStream_t st, const double *U, float *V, uin
     return hipGetLastError();
 }
 
+hipError_t launch_apply_mx(dim3 g, hipStream_t st, const DevMesh &m, const ApplyArgs &a) {
+    hipLaunchKernelGGL(k_apply_mx, g, dim3(BLOCK), 0, st, m, a);
+    return hipGetLastError();
+}
+
 }  // namespace mx
 }  // namespace nf

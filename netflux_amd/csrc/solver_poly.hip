@@ -300,10 +300,13 @@ __global__ __launch_bounds__(PB) void k_poly_step_dt(const double *part, uint32_
 }
 
 // The step's residual: sum of the block partials in a fixed order (each
-// thread a strided sum, then a tree), its square root, and the drop test
-// against ref (or this run's first residual when ref is 0).
+// thread a strided sum, then a tree), its square root, and the drop test.
+// runp = {drop, ref}: ref is the first residual since the state was set, or
+// negative when this run's first step is that residual (the host loop's
+// hist[0]; a first residual of 0 never converges, as there).  Read from
+// device memory so a captured graph serves any drop and reference.
 __global__ __launch_bounds__(PB) void k_poly_step_res(const double *part, uint32_t nb, int *flags, double *hres,
-                                                      double drop, double ref) {
+                                                      const double *runp) {
     __shared__ double sh[PB];
     if (flags[0]) return;
     const int k = flags[3];
@@ -319,7 +322,8 @@ __global__ __launch_bounds__(PB) void k_poly_step_res(const double *part, uint32
         const double r = sqrt(sh[0]);
         hres[k] = r;
         flags[3] = k + 1;
-        const double r0 = ref > 0.0 ? ref : hres[0];
+        const double drop = runp[0], ref = runp[1];
+        const double r0 = ref >= 0.0 ? ref : hres[0];
         if (drop > 0.0 && r0 > 0.0 && r / r0 <= drop) {
             flags[1] = 1;
             flags[0] = 1;
@@ -351,8 +355,11 @@ struct PolyPath {
     int *flags = nullptr;   // device-resident run flags
     double *dtdev = nullptr, *dts = nullptr, *hres = nullptr;  // its dt, and per step dt and residual
     int run_cap = 0;
+    double *runp = nullptr;  // device {drop, ref} of the current run
+    hipGraphExec_t graph = nullptr;  // check_every steps of nf_run, captured once per K and buffers
+    int graph_K = 0;
     int *bad = nullptr;
-    bool aux_valid = false;
+    bool aux_valid = false;  // grad / psi / R hold nf_eval_flux_balance of the current state
     // run state since the last set_state / set_freestream
     // kernel timing (nf_time_kernels): HIP events around each launch
     bool timing = false;
@@ -365,6 +372,7 @@ struct PolyPath {
     std::vector<void *> owned;
     ~PolyPath() {
         if (st) (void)hipStreamSynchronize(st);
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : owned) (void)hipFree(p);
     }
 };
@@ -617,6 +625,7 @@ static void reset_run(PolyPath *P) {
     P->hist.clear();
     P->time = 0.0;
     P->converged = false;
+    P->aux_valid = false;
 }
 
 int poly_set_freestream(PolyPath *P, std::string &err) {
@@ -687,6 +696,7 @@ int poly_calc_timestep(PolyPath *P, double *dt_out, std::string &err) {
 // Solver::step (src/Solver.cpp:34-52, 83-133)
 int poly_step(PolyPath *P, double dt, std::string &err) {
     int rc;
+    P->aux_valid = false;
     const bool rk3 = P->prm.time_scheme == NF_TVD_RK3, lts = P->prm.local_time_step != 0;
     // the residual is the flux balance of U^n (first stage), as in the batched path
     if ((rc = flux_balance(P, P->V, 0, dt, false, true, lts, err))) return rc;
@@ -795,7 +805,7 @@ int poly_time_kernels(PolyPath *P, int32_t n_steps, nf_kernel_time *out, int32_t
 
 // One step of the device-resident run: the same kernels and values as
 // calc_timestep + step, with dt, residual and the stop test on the device.
-static int enqueue_run_step(PolyPath *P, double drop, double ref, std::string &err) {
+static int enqueue_run_step(PolyPath *P, std::string &err) {
     PolyDev m = P->m;
     m.stop = P->flags;
     const uint32_t nb = (m.N + PB - 1) / PB;
@@ -816,7 +826,7 @@ static int enqueue_run_step(PolyPath *P, double drop, double ref, std::string &e
     hipLaunchKernelGGL(k_poly_sumsq, dim3(nb), dim3(PB), 0, P->st, (const double *)P->Rrho, m.N, P->part,
                        (const int *)P->flags);
     hipLaunchKernelGGL(k_poly_step_res, dim3(1), dim3(PB), 0, P->st, (const double *)P->part, nb, P->flags,
-                       P->hres, drop, ref);
+                       P->hres, (const double *)P->runp);
     PHIP(hipGetLastError());
     return NF_OK;
 }
@@ -831,48 +841,60 @@ int poly_run(PolyPath *P, const nf_run_opts *o, nf_run_stats *stats, std::string
     if (P->prm.check_physical_validity) return poly_run_host(P, o, stats, err);
     const double drop = o->residual_drop;
     int rc;
-    if (!P->flags && ((rc = alloc(P, &P->flags, 4, err)) || (rc = alloc(P, &P->dtdev, 1, err)))) return rc;
-    if (o->n_steps > P->run_cap) {  // per-step history buffers (the old ones stay owned until destroy)
+    P->aux_valid = false;
+    if (!P->flags && ((rc = alloc(P, &P->flags, 4, err)) || (rc = alloc(P, &P->dtdev, 1, err)) ||
+                      (rc = alloc(P, &P->runp, 2, err))))
+        return rc;
+    if (o->n_steps > P->run_cap) {  // per-step history buffers, grown (the graph refers to them)
         const int cap = std::max(o->n_steps, 1024);
+        PHIP(hipStreamSynchronize(P->st));
+        if (P->graph) {
+            (void)hipGraphExecDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        for (double *old : {P->dts, P->hres}) {
+            if (!old) continue;
+            P->owned.erase(std::find(P->owned.begin(), P->owned.end(), (void *)old));
+            PHIP(hipFree(old));
+        }
+        P->dts = P->hres = nullptr;
         if ((rc = alloc(P, &P->dts, cap, err)) || (rc = alloc(P, &P->hres, cap, err))) return rc;
         P->run_cap = cap;
     }
     PHIP(hipMemsetAsync(P->flags, 0, 4 * sizeof(int), P->st));
+    // this run's drop and reference (negative: the run's own first residual)
+    const double runp[2] = {drop, P->hist.empty() ? -1.0 : P->hist.front()};
+    PHIP(hipMemcpyAsync(P->runp, runp, sizeof runp, hipMemcpyHostToDevice, P->st));
     hipEvent_t e0, e1;
     PHIP(hipEventCreate(&e0));
     PHIP(hipEventCreate(&e1));
     PHIP(hipEventRecord(e0, P->st));
-    const double ref = P->hist.empty() ? 0.0 : P->hist.front();
     const int K = std::max(1, o->check_every);
     int flags[4] = {0, 0, 0, 0}, queued = 0;
-    struct GraphExec {
-        hipGraphExec_t g = nullptr;
-        ~GraphExec() {
-            if (g) (void)hipGraphExecDestroy(g);
-        }
-    } ge;
-    hipGraphExec_t &graph = ge.g;
     if (!(drop > 0.0 && P->converged)) {
         while (queued < o->n_steps && !flags[0]) {
             const int chunk = std::min(K, o->n_steps - queued);
             if (o->use_graph && chunk == K && K > 1) {
-                if (!graph) {
+                if (!P->graph || P->graph_K != K) {
+                    if (P->graph) (void)hipGraphExecDestroy(P->graph);
+                    P->graph = nullptr;
                     hipGraph_t gr;
                     PHIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
                     for (int q = 0; q < K; ++q)
-                        if ((rc = enqueue_run_step(P, drop, ref, err))) {
+                        if ((rc = enqueue_run_step(P, err))) {
                             (void)hipStreamEndCapture(P->st, &gr);
                             return rc;
                         }
                     PHIP(hipStreamEndCapture(P->st, &gr));
-                    const hipError_t ie = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+                    const hipError_t ie = hipGraphInstantiate(&P->graph, gr, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(gr);
                     PHIP(ie);
+                    P->graph_K = K;
                 }
-                PHIP(hipGraphLaunch(graph, P->st));
+                PHIP(hipGraphLaunch(P->graph, P->st));
             } else {
                 for (int q = 0; q < chunk; ++q)
-                    if ((rc = enqueue_run_step(P, drop, ref, err))) return rc;
+                    if ((rc = enqueue_run_step(P, err))) return rc;
             }
             queued += chunk;
             PHIP(hipMemcpyAsync(flags, P->flags, sizeof flags, hipMemcpyDeviceToHost, P->st));

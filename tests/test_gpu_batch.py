@@ -135,3 +135,38 @@ def test_instances_stop_independently_at_residual_drop():
     # a converged instance stays stopped when the run continues with a drop
     again = b.run(8, residual_drop=drop, check_every=4)
     assert again[a]["steps_done"] == 0 and again[a]["converged"] == 1
+
+
+def test_bump_channel_batch_mixed_then_fp64():
+    """The batched workload's shape, scaled down: 16 bump channels of uneven
+    size (log-uniform 3k-40k cells), Mach U[0.3, 0.8] and bump height
+    U[0.02, 0.1] from a seed, local time steps, mixed precision then FP64 in
+    the same context, launch picked by the autotuner.  Every instance ends
+    bit-identical to its mesh advanced alone through the same phases."""
+    from netflux_amd import steady as SD
+    cases = SD.sample_cases(16, seed=11, cells_lo=3e3, cells_hi=4e4)
+    meshes = [SD.mesh_of(c) for c in cases]
+    cfgs = [SD.config_of(c, "mixed") for c in cases]
+    U0 = [SD.freestream_state(cf, m.n_cells) for cf, m in zip(cfgs, meshes)]
+    print(f"[case] 16 bump channels, {sum(m.n_cells for m in meshes)} cells, mixed 30 + fp64 20 steps", flush=True)
+    assert len({m.n_cells for m in meshes}) > 4
+    b = nf.BatchSolver(cfgs, meshes)
+    for i, u in enumerate(U0):
+        b.set_state(i, u, u)
+    tune = b.autotune(steps=4)
+    print("launch", tune, flush=True)
+    st1 = b.run(30, check_every=10)
+    b.set_precision("fp64")
+    st2 = b.run(20, check_every=10)
+    assert all(s["steps_done"] == 30 for s in st1) and all(s["steps_done"] == 20 for s in st2)
+    for i, (m, cf, u) in enumerate(zip(meshes, cfgs, U0)):
+        s = nf.EulerSolver(cf, m)
+        s.set_state(u, u)
+        s.run(30, check_every=10)
+        s.set_precision("fp64")
+        s.run(20, check_every=10)
+        Us = s.get_solver_data().get_solution()
+        Ub = b.get_solution(i)
+        assert np.abs(Us - u).max() > 1e-6  # the flow has developed
+        np.testing.assert_array_equal(Ub, Us)
+        np.testing.assert_allclose(b.residuals(i), s.residuals(), rtol=1e-13, atol=0)

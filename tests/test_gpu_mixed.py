@@ -328,3 +328,53 @@ def test_mixed_local_time_steps_reach_the_same_steady_state():
     U_g, U_l = out[False][1], out[True][1]
     assert rel_l2(U_l, U_g) < 1e-5
     assert rel_l2(U_g, np.tile(U_g[0], (len(U_g), 1))) > 1e-3   # a non-trivial flow
+
+
+def test_mixed_continued_run_with_zero_first_residual():
+    """At rest with uniform density and slip walls the first step's density
+    residual is exactly 0.  A continued run with a drop measures against that
+    first residual, as the Driver loop on the host does, and so never stops
+    early; the device-resident loop and the host loop (the flux-balance check
+    on) agree step for step."""
+    mesh = mixed_box((4, 3, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=33)
+    V = np.zeros((mesh.n_cells, 5))
+    V[:, 0] = 1.0
+    V[:, 4] = 1.0
+    from tests.cases import prim_to_cons
+    U0 = prim_to_cons(V)
+    runs = []
+    for chk in (False, True):
+        s = nf.EulerSolver(sod_config(check_physical_validity=chk), mesh)
+        s.set_state(U0)
+        first = s.run(1)
+        assert first["res_first"] == 0.0
+        st = s.run(6, residual_drop=0.5, check_every=3)
+        runs.append((st, s.residuals(), s.get_solver_data().get_solution()))
+    for st, r, _ in runs:
+        assert st["steps_done"] == 6 and not st["converged"] and r[0] == 0.0 and len(r) == 7
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+
+
+def test_mixed_aux_fields_follow_the_state():
+    """Gradient, limiter and flux balance are exported only for the state
+    nf_eval_flux_balance saw: setting a state, stepping or running makes them
+    unavailable until the next evaluation."""
+    mesh = mixed_box((4, 3, 3), kinds=("hex", "wedge", "pyr"), jitter=0.1, seed=34)
+    s = nf.EulerSolver(sod_config(), mesh)
+    U0 = sod_state(mesh)
+    for change in ("set_state", "step", "run"):
+        s.set_state(U0)
+        s.eval_flux_balance()
+        s.get_solver_data().get_primvars_limiter()
+        if change == "set_state":
+            s.set_state(U0)
+        elif change == "step":
+            cfg = sod_config()
+            s.calc_timestep(cfg)
+            s.step(cfg)
+        else:
+            s.run(2)
+        for get in ("get_primvars_limiter", "get_primvars_gradient", "get_flux_balance"):
+            with pytest.raises(RuntimeError, match="nf_eval_flux_balance first"):
+                getattr(s.get_solver_data(), get)()
