@@ -307,6 +307,16 @@ int nf_mlp_set_norm(nf_mlp *m, const float *in_shift, const float *in_scale, con
 int nf_mlp_get_norm(nf_mlp *m, float *in_shift, float *in_scale, float *out_shift, float *out_scale);
 /* raw features x [n][6] -> raw outputs y [n][5] (host arrays) */
 int nf_mlp_predict(nf_mlp *m, const float *x, int64_t n, float *y);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): nf_mlp_predict on
+ * DEVICE arrays x [n][6] -> y [n][5], at most 16384 samples per call (more:
+ * NF_E_ARG), enqueued on `stream` (a hipStream_t; null: the network's own
+ * stream) without synchronising and without allocating.  The activations
+ * live in a fixed device array of about 17 MB per GPU, shared by every call
+ * on that GPU: calls that overlap on one GPU race.  The device that was
+ * current at nf_mlp_create must be current.  Same results as nf_mlp_predict,
+ * bit for bit. */
+int nf_mlp_predict_noalloc(nf_mlp *m, const float *x, int64_t n, float *y, void *stream);
 /* mean squared error of the normalised outputs over one batch of n <= 2^20
  * samples and its gradient with respect to the FP32 parameters */
 int nf_mlp_loss_grad(nf_mlp *m, const float *x, const float *y, int64_t n, float *loss, float *grad);

@@ -141,6 +141,7 @@ SIGNATURES = [
     ("nf_mlp_set_norm", C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p]),
     ("nf_mlp_get_norm", C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p]),
     ("nf_mlp_predict", C.c_int, [C.c_void_p, f32p, C.c_int64, f32p]),
+    ("nf_mlp_predict_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     ("nf_mlp_loss_grad", C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, f32p, f32p]),
     ("nf_mlp_train", C.c_int, [C.c_void_p, f32p, f32p, C.c_int64, C.POINTER(TrainOpts),
                                C.POINTER(TrainStats)]),

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <type_traits>
@@ -258,6 +259,18 @@ __global__ void k_denorm(const float *Y, int n, const float *out_shift, const fl
     if (j >= n) return;
     for (int o = 0; o < OUT; ++o) y[o * n_all + first + j] = Y[(size_t)o * n + j] * out_scale[o] + out_shift[o];
 }
+// the same normalisation from / de-normalisation to sample-major [n][w]
+// arrays (the layout of nf_mlp_predict's host arrays)
+__global__ void k_gather_aos(const float *x, int n, const float *in_shift, const float *in_scale, __bf16 *X0) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int f = 0; f < IN; ++f) X0[(size_t)f * n + j] = (__bf16)((x[(size_t)j * IN + f] - in_shift[f]) * in_scale[f]);
+}
+__global__ void k_denorm_aos(const float *Y, int n, const float *out_shift, const float *out_scale, float *y) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int o = 0; o < OUT; ++o) y[(size_t)j * OUT + o] = Y[(size_t)o * n + j] * out_scale[o] + out_shift[o];
+}
 
 // Adam step on the FP32 masters (bias-corrected step size lr_t), BF16 copy
 __global__ void k_adam(float *p, const float *g, float *m, float *v, __bf16 *pb, int64_t n, float lr_t, float b1,
@@ -297,6 +310,7 @@ struct nf_mlp {
     float *Z[3] = {nullptr, nullptr, nullptr}, *Y = nullptr, *T = nullptr, *dY = nullptr, *dA = nullptr;
     float *part = nullptr, *loss_part = nullptr;
     std::vector<void *> ws;
+    int device = 0;  // device current at nf_mlp_create, holding the parameters
 };
 
 namespace {
@@ -359,21 +373,36 @@ const float *d_out_shift(const nf_mlp *m) { return m->norm + 12; }
 const float *d_out_scale(const nf_mlp *m) { return m->norm + 17; }
 const float *d_out_iscale(const nf_mlp *m) { return m->norm + 22; }
 
+// activation buffers of one forward pass, feature-major [rows][n]
+struct FwdBufs {
+    const __bf16 *X0;
+    float *Z[3];
+    __bf16 *A[3];
+    float *Y;
+};
+
 // forward of the n samples in X0 (normalised); Y = normalised outputs
-hipError_t forward(nf_mlp *m, int n, hipStream_t s) {
+hipError_t forward_on(const nf_mlp *m, const FwdBufs &b, int n, hipStream_t s) {
     hipError_t e;
-    const __bf16 *in = m->X0;
+    const __bf16 *in = b.X0;
     int k = IN;
     const int64_t offW[3] = {OFF_W1, OFF_W2, OFF_W3}, offB[3] = {OFF_B1, OFF_B2, OFF_B3};
     for (int l = 0; l < 3; ++l) {
-        if ((e = gemm(s, m->pb + offW[l], k, 1, in, n, 1, m->Z[l], n, 1, HID, n, k))) return e;
-        hipLaunchKernelGGL(k_bias_gelu, dim3((n + 255) / 256, HID), dim3(256), 0, s, m->Z[l],
-                           (const float *)(m->p + offB[l]), m->A[l], HID, n);
+        if ((e = gemm(s, m->pb + offW[l], k, 1, in, n, 1, b.Z[l], n, 1, HID, n, k))) return e;
+        hipLaunchKernelGGL(k_bias_gelu, dim3((n + 255) / 256, HID), dim3(256), 0, s, b.Z[l],
+                           (const float *)(m->p + offB[l]), b.A[l], HID, n);
         if ((e = hipGetLastError())) return e;
-        in = m->A[l];
+        in = b.A[l];
         k = HID;
     }
-    return gemm(s, m->pb + OFF_W4, HID, 1, in, n, 1, m->Y, n, 1, OUT, n, HID);
+    return gemm(s, m->pb + OFF_W4, HID, 1, in, n, 1, b.Y, n, 1, OUT, n, HID);
+}
+
+// forward over the network's own workspace (training keeps every layer's
+// pre-activations and activations for the backward pass)
+hipError_t forward(nf_mlp *m, int n, hipStream_t s) {
+    const FwdBufs b{m->X0, {m->Z[0], m->Z[1], m->Z[2]}, {m->A[0], m->A[1], m->A[2]}, m->Y};
+    return forward_on(m, b, n, s);
 }
 
 // loss and gradient (into m->g) of the n samples in X0 / T
@@ -473,9 +502,91 @@ hipError_t predict_device(nf_mlp *m, const float *x, int64_t n, float *y, hipStr
 }  // namespace nf
 
 // ============================================================================
+// ---------------------------------------------------------------------------
+// Inference that allocates nothing, for callers that may not call hipMalloc
+// (code under HIP graph capture, serving loops).  The activations of one call
+// -- normalised inputs X0 (BF16 [6][n]), one layer's pre-activations Z (FP32
+// [128][n]), two BF16 activation buffers [128][n] used in turn, and the
+// outputs Y (FP32 [5][n]) -- live in `scratch`, a file-scope device array
+// sized for NOALLOC_MAX samples (about 17 MB).  Each GPU the library runs on
+// holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: nf_mlp_create looks up the copy of the current
+// device once (hipGetSymbolAddress) and the kernels take that address.  Every
+// buffer is written in full before it is read, so nothing needs clearing
+// between calls; calls that overlap on one GPU (two streams or threads)
+// share the array and race.
+namespace {
+
+constexpr int64_t NOALLOC_MAX = 16384;  // samples per call
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_X0 = 0;
+constexpr size_t NA_Z = al256(NA_X0 + sizeof(__bf16) * IN * NOALLOC_MAX);
+constexpr size_t NA_A0 = al256(NA_Z + sizeof(float) * HID * NOALLOC_MAX);
+constexpr size_t NA_A1 = al256(NA_A0 + sizeof(__bf16) * HID * NOALLOC_MAX);
+constexpr size_t NA_Y = al256(NA_A1 + sizeof(__bf16) * HID * NOALLOC_MAX);
+constexpr size_t NA_BYTES = al256(NA_Y + sizeof(float) * OUT * NOALLOC_MAX);
+
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned char *scratch_ptr[MAX_DEVICES] = {};  // per device: its copy of `scratch`
+
+// once per device, with that device current (called by nf_mlp_create)
+int scratch_setup(nf_mlp *mm, int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(mm, NF_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_ptr[dev]) return NF_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(mm, NF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    return NF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t nf_mlp_n_params(void) { return NPARAM; }
+
+int nf_mlp_predict_noalloc(nf_mlp *mm, const float *x, int64_t n, float *y, void *stream) {
+    if (!mm || !x || !y || n < 0) return NF_E_ARG;
+    if (n > NOALLOC_MAX)
+        return fail(mm, NF_E_ARG,
+                    "nf_mlp_predict_noalloc: " + std::to_string(n) + " samples exceed the maximum of " +
+                        std::to_string(NOALLOC_MAX) + " per call");
+    if (n == 0) return NF_OK;
+    int dev = -1;
+    HIPM(hipGetDevice(&dev));
+    if (dev != mm->device)
+        return fail(mm, NF_E_ARG, "nf_mlp_predict_noalloc: the network's device must be current");
+    unsigned char *base;
+    {
+        std::lock_guard<std::mutex> lk(scratch_mu);
+        base = scratch_ptr[dev];
+    }
+    if (!base) return fail(mm, NF_E_HIP, "nf_mlp_predict_noalloc: scratch not set up on this device");
+    const int c = (int)n;
+    __bf16 *X0 = reinterpret_cast<__bf16 *>(base + NA_X0);
+    float *Z = reinterpret_cast<float *>(base + NA_Z);
+    __bf16 *A0 = reinterpret_cast<__bf16 *>(base + NA_A0), *A1 = reinterpret_cast<__bf16 *>(base + NA_A1);
+    float *Y = reinterpret_cast<float *>(base + NA_Y);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : mm->stream;
+    const dim3 g((c + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_gather_aos, g, b, 0, s, x, c, d_in_shift(mm), d_in_scale(mm), X0);
+    HIPM(hipGetLastError());
+    // layer l reads A[l-1] and writes A[l]: A0, A1, A0
+    const FwdBufs fb{X0, {Z, Z, Z}, {A0, A1, A0}, Y};
+    HIPM(forward_on(mm, fb, c, s));
+    hipLaunchKernelGGL(k_out_loss, g, b, 0, s, Y, (const float *)(mm->p + OFF_B4), (const float *)nullptr,
+                       (float *)nullptr, (__bf16 *)nullptr, (float *)nullptr, c, 0.f);
+    HIPM(hipGetLastError());
+    hipLaunchKernelGGL(k_denorm_aos, g, b, 0, s, (const float *)Y, c, d_out_shift(mm), d_out_scale(mm), y);
+    HIPM(hipGetLastError());
+    return NF_OK;
+}
 
 int nf_mlp_create(uint64_t seed, nf_mlp **out) {
     if (!out) return NF_E_ARG;
@@ -484,6 +595,9 @@ int nf_mlp_create(uint64_t seed, nf_mlp **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NF_E_NOGPU;
     std::unique_ptr<nf_mlp> hold(new nf_mlp);
     nf_mlp *mm = hold.get();
+    HIPM(hipGetDevice(&mm->device));
+    int rc0 = scratch_setup(mm, mm->device);
+    if (rc0) return rc0;
     HIPM(hipStreamCreateWithFlags(&mm->stream, hipStreamNonBlocking));
     HIPM(hipMalloc(&mm->p, NPARAM * sizeof(float)));
     HIPM(hipMalloc(&mm->g, NPARAM * sizeof(float)));
