@@ -42,10 +42,22 @@ def bytes_model(N, F):
     return sweep1, sweep2_s1, sweep2_s23, step
 
 
-def profiled_traffic(kernel, n):
+def bytes_model_mixed(N, F):
+    """The same compulsory-traffic model with FP32 work arrays and geometry
+    (FP64 state): a face record is 44 B, limited slopes 60 B per cell, volumes
+    stay 8 B.  Sweep 1 = 108 N + 44 F, sweep 2 = 148 N + 44 F (stage 1) or
+    188 N + 44 F (with U_old); per RK3 step 860 N + 264 F."""
+    sweep1 = 108 * N + 44 * F
+    sweep2_s1 = 148 * N + 44 * F
+    sweep2_s23 = 188 * N + 44 * F
+    step = 860 * N + 264 * F
+    return sweep1, sweep2_s1, sweep2_s23, step
+
+
+def profiled_traffic(kernel, n, fname="sweep_traffic_128cubed.md"):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the 128^3 workload."""
-    path = os.path.join(ROOT, "profiles", "sweep_traffic_128cubed.md")
+    path = os.path.join(ROOT, "profiles", fname)
     if n != 128 or not os.path.exists(path):
         return None
     for line in open(path):
@@ -116,6 +128,30 @@ def cpu_baseline(sample_n, steps):
             "sample": f"oracle, {sample_n}^3x6 = {N} cells, {steps} TVD-RK3 steps, same workload family "
                       f"(jitter 0.15, pulse, farfield), one thread pinned to core {cpus[0]} of "
                       f"{len(cpus)} available, {model}, {t:.1f} s"}
+
+
+def steady_block(args):
+    """The metric's second half: time to a 1e-8 density-residual drop per
+    solve, learned + batched against plain FP64, on bump channels of
+    20k-100k cells (netflux_amd.steady.time_to_drop).  The network is the
+    committed one, trained by the library's kernels on plain-solver steady
+    solutions of other cases."""
+    from netflux_amd import steady as SD
+    if not os.path.exists(args.steady_net):
+        return {"skipped": f"no network at {args.steady_net}"}
+    net, meta = SD.load_network(args.steady_net)
+    t0 = time.perf_counter()
+    rep = SD.time_to_drop(net, n_cases=args.steady_cases)
+    print(f"[bench] steady comparison {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    rep["metric"] = "time-to-1e-8 density-residual drop per solve"
+    rep["unit"] = "s per instance"
+    rep["higher_is_better"] = False
+    rep["network"] = {"file": os.path.relpath(args.steady_net, ROOT), "trained_on": meta}
+    rep["config"] = ("bump channel [0,3]x[0,1]x[0,0.5] (Gaussian floor bump, sigma 0.2), cells log-uniform "
+                     "20k-100k, Mach U[0.3,0.7], h U[0.02,0.1], farfield x-planes, slip walls, unlimited "
+                     "second order, CFL 1.0, TVD-RK3, U_old = U^0; drop measured against each case's "
+                     "freestream residual")
+    return rep
 
 
 def poly_bytes_model(N, F, G):
@@ -204,10 +240,14 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--order", default="morton", choices=["input", "rcm", "morton"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="mixed: FP32 face arithmetic with FP64 state (not the headline; no roofline)")
+                    help="mixed: FP32 face arithmetic with FP64 state (not the headline)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches in the timed region (profiling)")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
                     help="pick the sweep launch configuration with the autotuner first")
+    ap.add_argument("--steady-cases", type=int, default=8,
+                    help="bump channels of the time-to-1e-8 comparison (0: skip it)")
+    ap.add_argument("--steady-net", default=os.path.join(ROOT, "netflux_amd", "data", "warm_start_bump.npz"),
+                    help="warm-start network for the time-to-1e-8 comparison")
     ap.add_argument("--mesh", default="tet", choices=["tet", "hex"],
                     help="hex: an n^3 hexahedral mesh through the mixed-element (polyhedral) path")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -265,33 +305,41 @@ def main():
 
     # per-kernel durations with HIP events on the solver's stream
     kt = solver.time_kernels(args.timing_steps)
-    s1, s2a, s2b, step_bytes = bytes_model(N, F)
+    mixed = args.precision == "mixed"
+    s1, s2a, s2b, step_bytes = (bytes_model_mixed if mixed else bytes_model)(N, F)
     kern = {}
     for name, (avg, calls) in kt.items():
         kern[name] = {"avg_ms": avg, "calls": calls}
     tot = {k: v["avg_ms"] * v["calls"] for k, v in kern.items()}
     dom = max(tot, key=tot.get)
-    if args.precision == "fp64":
-        per_launch = {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0}.get(dom, s1)
-        achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.n),
-                    "traffic_source": "profiles/sweep_traffic_128cubed.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
-                    "kernel": dom,
-                    "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
-                    "algorithmic_bytes_per_launch": int(per_launch)}
-        if args.copy_reference:
-            cp = measured_copy_gbs()
-            if cp:
-                roofline["measured_copy_GBs"] = round(cp, 1)
-                roofline["frac_of_measured_copy"] = round(achieved / cp, 4)
-        step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
-    else:
-        roofline, step_frac = None, None  # the byte model above is the FP64 one
+    sweeps = ({"sweepA_barth_mx": s1, "sweepB_mx": (s2a + 2 * s2b) / 3.0} if mixed
+              else {"sweepA_barth": s1, "sweepB": (s2a + 2 * s2b) / 3.0})
+    per_launch = sweeps.get(dom, s1)
+    achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
+    traffic_file = "sweep_traffic_mixed_128cubed.md" if mixed else "sweep_traffic_128cubed.md"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.n, traffic_file),
+                "traffic_source": f"profiles/{traffic_file} (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                "kernel": dom,
+                "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
+                "algorithmic_bytes_per_launch": int(per_launch),
+                "other_sweep": {k: {"algorithmic_bytes_per_launch": int(v),
+                                    "frac": round(v / (kern[k]["avg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                for k, v in sweeps.items() if k != dom and k in kern}}
+    if args.copy_reference:
+        cp = measured_copy_gbs()
+        if cp:
+            roofline["measured_copy_GBs"] = round(cp, 1)
+            roofline["frac_of_measured_copy"] = round(achieved / cp, 4)
+    step_frac = step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample_n, args.cpu_steps)
+    steady = None
+    if args.steady_cases and args.precision == "fp64":
+        del solver  # frees the 128^3 context first
+        steady = steady_block(args)
 
     out = {
         "metric": "cell-stage-updates/s",
@@ -314,6 +362,7 @@ def main():
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "steady": steady,
     }
     print(json.dumps(out))
 

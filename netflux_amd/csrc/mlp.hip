@@ -311,6 +311,11 @@ struct nf_mlp {
     float *part = nullptr, *loss_part = nullptr;
     std::vector<void *> ws;
     int device = 0;  // device current at nf_mlp_create, holding the parameters
+    // feature-major device copies of the caller's host arrays (inputs [6][n],
+    // targets or outputs [5][n], minibatch order), kept and grown as needed
+    float *dx = nullptr, *dy = nullptr;
+    int64_t *didx = nullptr;
+    int64_t data_cap = 0;
 };
 
 namespace {
@@ -462,13 +467,29 @@ int upload_norm(nf_mlp *mm) {
     return NF_OK;
 }
 
-// feature-major device copy of a host AoS [n][w] array
-int upload_fm(nf_mlp *mm, const float *aos, int64_t n, int w, float **dev) {
+// room for n samples in the data buffers (allocates only when n grows)
+int ensure_data(nf_mlp *mm, int64_t n) {
+    if (n <= mm->data_cap) return NF_OK;
+    HIPM(hipStreamSynchronize(mm->stream));
+    for (void *q : {(void *)mm->dx, (void *)mm->dy, (void *)mm->didx})
+        if (q) HIPM(hipFree(q));
+    mm->dx = mm->dy = nullptr;
+    mm->didx = nullptr;
+    mm->data_cap = 0;
+    const int64_t cap = (n + 255) / 256 * 256;
+    HIPM(hipMalloc(&mm->dx, (size_t)IN * cap * sizeof(float)));
+    HIPM(hipMalloc(&mm->dy, (size_t)OUT * cap * sizeof(float)));
+    HIPM(hipMalloc(&mm->didx, (size_t)cap * sizeof(int64_t)));
+    mm->data_cap = cap;
+    return NF_OK;
+}
+
+// a host AoS [n][w] array into a feature-major device buffer [w][n]
+int upload_fm(nf_mlp *mm, const float *aos, int64_t n, int w, float *dev) {
     std::vector<float> fm((size_t)w * n);
     for (int64_t j = 0; j < n; ++j)
         for (int f = 0; f < w; ++f) fm[(size_t)f * n + j] = aos[(size_t)j * w + f];
-    HIPM(hipMalloc(dev, fm.size() * sizeof(float)));
-    HIPM(hipMemcpy(*dev, fm.data(), fm.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPM(hipMemcpy(dev, fm.data(), fm.size() * sizeof(float), hipMemcpyHostToDevice));
     return NF_OK;
 }
 
@@ -629,7 +650,8 @@ int nf_mlp_destroy(nf_mlp *mm) {
     if (!mm) return NF_OK;
     if (mm->stream) (void)hipStreamSynchronize(mm->stream);
     for (void *q : mm->ws) (void)hipFree(q);
-    for (void *q : {(void *)mm->p, (void *)mm->g, (void *)mm->m, (void *)mm->v, (void *)mm->pb, (void *)mm->norm})
+    for (void *q : {(void *)mm->p, (void *)mm->g, (void *)mm->m, (void *)mm->v, (void *)mm->pb, (void *)mm->norm,
+                    (void *)mm->dx, (void *)mm->dy, (void *)mm->didx})
         if (q) (void)hipFree(q);
     if (mm->stream) (void)hipStreamDestroy(mm->stream);
     delete mm;
@@ -684,17 +706,13 @@ int nf_mlp_get_norm(nf_mlp *mm, float *in_shift, float *in_scale, float *out_shi
 int nf_mlp_predict(nf_mlp *mm, const float *x, int64_t n, float *y) {
     if (!mm || !x || !y || n < 0) return NF_E_ARG;
     if (n == 0) return NF_OK;
-    float *dx = nullptr, *dy = nullptr;
-    int rc = upload_fm(mm, x, n, IN, &dx);
-    if (rc) return rc;
-    HIPM(hipMalloc(&dy, (size_t)OUT * n * sizeof(float)));
-    const hipError_t e = nf::mlp::predict_device(mm, dx, n, dy, mm->stream);
-    std::vector<float> fm((size_t)OUT * n);
-    if (e == hipSuccess && hipStreamSynchronize(mm->stream) == hipSuccess)
-        (void)hipMemcpy(fm.data(), dy, fm.size() * sizeof(float), hipMemcpyDeviceToHost);
-    (void)hipFree(dx);
-    (void)hipFree(dy);
+    int rc;
+    if ((rc = ensure_data(mm, n)) || (rc = upload_fm(mm, x, n, IN, mm->dx))) return rc;
+    const hipError_t e = nf::mlp::predict_device(mm, mm->dx, n, mm->dy, mm->stream);
     if (e != hipSuccess) return fail(mm, NF_E_HIP, std::string("predict: ") + hipGetErrorString(e));
+    std::vector<float> fm((size_t)OUT * n);
+    HIPM(hipStreamSynchronize(mm->stream));
+    HIPM(hipMemcpy(fm.data(), mm->dy, fm.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < n; ++j)
         for (int o = 0; o < OUT; ++o) y[(size_t)j * OUT + o] = fm[(size_t)o * n + j];
     return NF_OK;
@@ -704,20 +722,16 @@ int nf_mlp_predict(nf_mlp *mm, const float *x, int64_t n, float *y) {
 int nf_mlp_loss_grad(nf_mlp *mm, const float *x, const float *y, int64_t n, float *loss, float *grad) {
     if (!mm || !x || !y || n < 1 || n > (1 << 20)) return NF_E_ARG;
     int rc;
-    if ((rc = ensure_ws(mm, (int)n))) return rc;
-    float *dx = nullptr, *dy = nullptr;
-    if ((rc = upload_fm(mm, x, n, IN, &dx))) return rc;
-    if ((rc = upload_fm(mm, y, n, OUT, &dy))) return rc;
+    if ((rc = ensure_ws(mm, (int)n)) || (rc = ensure_data(mm, n))) return rc;
+    if ((rc = upload_fm(mm, x, n, IN, mm->dx)) || (rc = upload_fm(mm, y, n, OUT, mm->dy))) return rc;
     const int c = (int)n;
-    hipLaunchKernelGGL(k_gather, dim3((c + 255) / 256), dim3(256), 0, mm->stream, (const float *)dx, (const float *)dy,
-                       n, (const int64_t *)nullptr, (int64_t)0, c, d_in_shift(mm), d_in_scale(mm), d_out_shift(mm),
-                       d_out_iscale(mm), mm->X0, mm->T);
+    hipLaunchKernelGGL(k_gather, dim3((c + 255) / 256), dim3(256), 0, mm->stream, (const float *)mm->dx,
+                       (const float *)mm->dy, n, (const int64_t *)nullptr, (int64_t)0, c, d_in_shift(mm),
+                       d_in_scale(mm), d_out_shift(mm), d_out_iscale(mm), mm->X0, mm->T);
     hipError_t e = hipGetLastError();
     if (!e) e = forward(mm, c, mm->stream);
     if (!e) e = backward(mm, c, mm->stream);
     if (!e) e = hipStreamSynchronize(mm->stream);
-    (void)hipFree(dx);
-    (void)hipFree(dy);
     if (e != hipSuccess) return fail(mm, NF_E_HIP, std::string("loss_grad: ") + hipGetErrorString(e));
     if (loss) *loss = sum_loss(mm, c);
     if (grad) HIPM(hipMemcpy(grad, mm->g, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
@@ -753,11 +767,10 @@ int nf_mlp_train(nf_mlp *mm, const float *x, const float *y, int64_t n, const nf
         if ((rc = upload_norm(mm))) return rc;
         mm->norm_set = true;
     }
-    float *dx = nullptr, *dy = nullptr;
-    int64_t *didx = nullptr;
-    if ((rc = upload_fm(mm, x, n, IN, &dx))) return rc;
-    if ((rc = upload_fm(mm, y, n, OUT, &dy))) return rc;
-    HIPM(hipMalloc(&didx, n * sizeof(int64_t)));
+    if ((rc = ensure_data(mm, n))) return rc;
+    if ((rc = upload_fm(mm, x, n, IN, mm->dx)) || (rc = upload_fm(mm, y, n, OUT, mm->dy))) return rc;
+    const float *dx = mm->dx, *dy = mm->dy;
+    int64_t *didx = mm->didx;
     std::vector<int64_t> idx(n);
     for (int64_t j = 0; j < n; ++j) idx[j] = j;
     std::mt19937_64 rng(o->seed);
@@ -806,9 +819,6 @@ int nf_mlp_train(nf_mlp *mm, const float *x, const float *y, int64_t n, const nf
     if (!e) (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(dx);
-    (void)hipFree(dy);
-    (void)hipFree(didx);
     if (e != hipSuccess) return fail(mm, NF_E_HIP, std::string("train: ") + hipGetErrorString(e));
     if (st) {
         st->loss_first = first;

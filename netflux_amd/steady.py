@@ -233,6 +233,45 @@ def train_network(x, y, epochs=200, batch=2048, lr=2e-3, lr_decay=0.98, seed=0):
     return net, stats
 
 
+def save_network(net, path, meta=None):
+    """Parameters, normalisation and a JSON description in one .npz file."""
+    import json
+    sh, sc, osh, osc = net.get_norm()
+    np.savez(path, params=net.params, in_shift=sh, in_scale=sc, out_shift=osh, out_scale=osc,
+             meta=np.array(json.dumps(meta or {})))
+
+
+def load_network(path):
+    """A network saved by save_network (plain arrays; nothing is unpickled)."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        net = Mlp(0)
+        net.params = z["params"]
+        net.set_norm(z["in_shift"], z["in_scale"], z["out_shift"], z["out_scale"])
+        meta = json.loads(str(z["meta"]))
+    return net, meta
+
+
+def train_from_solves(train_cases, train_cells, seed, per_case=4000, train_drop=1e-6, max_steps=60000,
+                      check_every=200, epochs=200):
+    """Steady FP64 solutions of `train_cases` sampled cases of about
+    `train_cells` cells (batched, to `train_drop`), `per_case` cells of each as
+    samples, and a network trained on them by the library's kernels."""
+    tcases = sample_cases(train_cases, seed, train_cells, train_cells)
+    t0 = time.perf_counter()
+    trun = BatchRun(tcases, precision="fp64")
+    tst, twall = trun.solve(train_drop, max_steps, check_every)
+    x, y = training_set(trun, per_case, tst, seed)
+    del trun
+    _log(f"[train data] {_summary(tst)} {twall:.2f} s, {len(x)} samples")
+    net, tstats = train_network(x, y, epochs=epochs)
+    _log(f"[train] {tstats}")
+    info = {"cases": train_cases, "cells": int(tcases[0].n_cells), "case_seed": seed, "drop": train_drop,
+            "solves": _summary(tst), "solve_wall_s": twall, "samples": int(len(x)), "epochs": epochs,
+            "train": tstats, "total_s": time.perf_counter() - t0}
+    return net, info
+
+
 def _log(msg):
     import sys
     print(msg, file=sys.stderr, flush=True)
@@ -249,12 +288,13 @@ def _summary(stats):
 
 def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60000, check_every=200,
           train_cases=0, train_cells=2e5, per_case=4000, epochs=200, train_drop=1e-6, plain=True,
-          plain_lts=False, partial=None, train_max_steps=60000):
-    """Plain FP64 solves one by one (the reference's global time step unless
-    plain_lts), the batched mixed-precision solve with local time steps, and
-    (train_cases > 0) the same batch from a learned warm start trained on
-    train_cases plain solutions of train_cells cells."""
-    cases = sample_cases(n_cases, seed, cells_lo, cells_hi)
+          plain_lts=False, partial=None, train_max_steps=60000, net_path=None, mach=(0.3, 0.8)):
+    """Plain FP64 solves one by one (the reference's global time step; with
+    plain_lts also, or only, with local time steps, which separates the gain
+    of local time stepping from that of batching), the batched FP64 and
+    mixed-precision solves with local time steps, and (train_cases > 0, or a
+    saved network at net_path) the same batches from a learned warm start."""
+    cases = sample_cases(n_cases, seed, cells_lo, cells_hi, mach=mach)
     rep = {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "plain_lts": plain_lts}
 
     def save():
@@ -264,21 +304,24 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
                 json.dump(rep, f)
 
     save()
-    plain_U, plain_wall, plain_stats = [], [], []
-    if plain:
+    plain_U = []
+    modes = {"both": [False, True], True: [True], False: [False]}[plain_lts] if plain else []
+    for lts in modes:
+        plain_wall, plain_stats, sols = [], [], []
         for i, c in enumerate(cases):
-            run = BatchRun([c], precision="fp64", local_time_step=plain_lts)
+            run = BatchRun([c], precision="fp64", local_time_step=lts)
             st, wall = run.solve(drop, max_steps, check_every)
             plain_stats.append(st[0])
             plain_wall.append(wall)
-            plain_U.append(run.solutions()[0])
-            _log(f"[plain] case {i} cells {c.n_cells} steps {st[0]['steps_done']} conv {st[0]['converged']} "
-                 f"{wall:.2f} s")
+            sols.append(run.solutions()[0])
+            _log(f"[plain{' lts' if lts else ''}] case {i} cells {c.n_cells} steps {st[0]['steps_done']} "
+                 f"conv {st[0]['converged']} {wall:.2f} s")
             del run
-        rep["plain_fp64"] = _summary(plain_stats) | {"wall_total_s": sum(plain_wall),
-                                                       "per_instance_s": sum(plain_wall) / n_cases,
-                                                       "steps": [s["steps_done"] for s in plain_stats],
-                                                       "wall_s": plain_wall}
+        if not plain_U:
+            plain_U = sols  # the first plain mode is the reference for the comparisons
+        rep["plain_fp64_lts" if lts else "plain_fp64"] = _summary(plain_stats) | {
+            "wall_total_s": sum(plain_wall), "per_instance_s": sum(plain_wall) / n_cases,
+            "steps": [s["steps_done"] for s in plain_stats], "wall_s": plain_wall}
         save()
 
     def compare(sols):
@@ -310,19 +353,15 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
     rep["batch_mixed"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases, "tune": run.tune,
                                          "steps": [s["steps_done"] for s in st]} | compare(run.solutions())
     save()
-    if train_cases:
-        tcases = sample_cases(train_cases, seed + 1000, train_cells, train_cells)
-        t0 = time.perf_counter()
-        trun = BatchRun(tcases, precision="fp64")
-        tst, twall = trun.solve(train_drop, train_max_steps, check_every)
-        x, y = training_set(trun, per_case, tst, seed)
-        del trun
-        _log(f"[train data] {_summary(tst)} {twall:.2f} s, {len(x)} samples")
-        net, tstats = train_network(x, y, epochs=epochs)
-        _log(f"[train] {tstats}")
-        rep["training"] = {"solves": _summary(tst), "solve_wall_s": twall, "samples": int(len(x)),
-                           "train": tstats, "total_s": time.perf_counter() - t0}
+    net = None
+    if net_path:
+        net, rep["training"] = load_network(net_path)
+        rep["training"]["file"] = net_path
+    elif train_cases:
+        net, rep["training"] = train_from_solves(train_cases, train_cells, seed + 1000, per_case, train_drop,
+                                                 train_max_steps, check_every, epochs)
         save()
+    if net is not None:
         st, wall = run.solve(drop, max_steps, check_every, net=net, stall_window=4000, stall_ratio=0.5)
         _log(f"[batch mixed warm] {_summary(st)} {wall:.2f} s")
         rep["batch_mixed_warm"] = _summary(st) | {"wall_s": wall, "per_instance_s": wall / n_cases,
@@ -334,6 +373,68 @@ def study(n_cases=8, seed=1, cells_lo=2e4, cells_hi=3e5, drop=1e-8, max_steps=60
                                                   "steps": [s["steps_done"] for s in st]} | compare(run64.solutions())
         save()
     return rep
+
+
+def time_to_drop(net, n_cases=8, seed=7, cells_lo=2e4, cells_hi=1e5, drop=1e-8, max_steps=120000,
+                 check_every=200, mach=(0.3, 0.7)):
+    """Time to a `drop` density-residual drop per instance, three ways, on
+    `n_cases` sampled bump channels:
+    plain    -- FP64, one context per case, the reference's global time step,
+                from the freestream (the reference's method on the GPU);
+    batched  -- all cases in one context, mixed precision to its FP32 floor
+                then FP64, local time steps, from the freestream;
+    learned  -- the same batch from the network's warm start.
+    Per-instance time is the batch wall time / the number of cases for the
+    batched modes.  Wall pressure of the learned solves is compared with the
+    plain ones.  A case whose plain solve does not reach the drop in
+    max_steps (a transonic bump: the unlimited scheme oscillates at the
+    shock, Barth stalls) is left out of the comparison and listed."""
+    sampled = sample_cases(n_cases, seed, cells_lo, cells_hi, mach=mach)
+    cases, excluded = [], []
+    plain_wall, plain_steps, plain_U = [], [], []
+    for c in sampled:
+        run = BatchRun([c], precision="fp64", local_time_step=False)
+        st, wall = run.solve(drop, max_steps, check_every)
+        if not st[0]["converged"]:
+            fd = st[0]["res_last"] / st[0]["r_ref"]
+            excluded.append(c.__dict__ | {"cells": c.n_cells, "final_drop": fd if math.isfinite(fd) else None})
+            continue
+        cases.append(c)
+        plain_wall.append(wall)
+        plain_steps.append(int(st[0]["steps_done"]))
+        plain_U.append(run.solutions()[0])
+        del run
+    n_cases = len(cases)
+    if n_cases == 0:
+        raise RuntimeError("no case reached the drop in the plain solve")
+    run = BatchRun(cases, precision="mixed", autotune_steps=20)
+    st_b, wall_b = run.solve(drop, max_steps, check_every)
+    st_l, wall_l = run.solve(drop, max_steps, check_every, net=net, stall_window=4000, stall_ratio=0.5)
+    for st in (st_b, st_l):
+        if not all(s["converged"] for s in st):
+            raise RuntimeError("a batched solve did not reach the drop")
+    dp = []
+    for i, c in enumerate(cases):
+        m = run.meshes[i]
+        _, p = wall_pressure(m, run.batch.get_solution(i))
+        _, p0 = wall_pressure(m, plain_U[i])
+        dp.append(float(np.max(np.abs(p - p0))))
+    plain_per = sum(plain_wall) / n_cases
+    return {
+        "cases": n_cases, "case_seed": seed, "mach_range": list(mach), "excluded": excluded,
+        "cells": [c.n_cells for c in cases],
+        "mach": [round(c.mach, 4) for c in cases], "bump_h": [round(c.h, 4) for c in cases], "drop": drop,
+        "plain_fp64": {"per_instance_s": plain_per, "wall_s": plain_wall, "steps": plain_steps},
+        "batched_freestream": {"per_instance_s": wall_b / n_cases, "wall_s": wall_b,
+                               "steps": [int(s["steps_done"]) for s in st_b], "launch": run.tune},
+        "learned_batched": {"per_instance_s": wall_l / n_cases, "wall_s": wall_l,
+                            "steps": [int(s["steps_done"]) for s in st_l],
+                            "fell_back": int(sum(s["fell_back"] for s in st_l))},
+        "speedup_per_instance": plain_per / (wall_l / n_cases),
+        "speedup_batched_freestream": plain_per / (wall_b / n_cases),
+        "warm_start_gain": wall_b / wall_l,
+        "wall_p_maxabs_diff_vs_plain": max(dp),
+    }
 
 
 def c3(drop=1e-8, max_steps=500000, check_every=200):
@@ -385,16 +486,28 @@ def main(argv=None):
     ap.add_argument("--per-case", type=int, default=4000)
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-lts", action="store_true")
+    ap.add_argument("--plain-lts", choices=["no", "yes", "both"], default="no",
+                    help="plain solves with local time steps too (both) or only (yes)")
+    ap.add_argument("--net", default=None, help="a saved network instead of training one")
+    ap.add_argument("--mach", type=float, nargs=2, default=(0.3, 0.8))
     ap.add_argument("--partial", default=None, help="write the report after each phase to this file")
     ap.add_argument("--c3", action="store_true", help="the 2M-tet mixed-precision channel case only")
+    ap.add_argument("--train-save", default=None, metavar="PATH",
+                    help="only train the warm-start network (train-cases at train-cells) and save it")
     a = ap.parse_args(argv)
+    if a.train_save:
+        net, info = train_from_solves(a.train_cases, a.train_cells, a.seed + 1000, a.per_case,
+                                      max_steps=a.max_steps, check_every=a.check_every, epochs=a.epochs)
+        save_network(net, a.train_save, info)
+        print(json.dumps(info))
+        return
     if a.c3:
         print(json.dumps(c3(a.drop, a.max_steps, a.check_every)))
         return
     rep = study(a.cases, a.seed, a.cells_lo, a.cells_hi, a.drop, a.max_steps, a.check_every, a.train_cases,
-                a.train_cells, a.per_case, a.epochs, plain=not a.no_plain, plain_lts=a.plain_lts,
-                partial=a.partial)
+                a.train_cells, a.per_case, a.epochs, plain=not a.no_plain,
+                plain_lts={"no": False, "yes": True, "both": "both"}[a.plain_lts], partial=a.partial,
+                train_max_steps=a.max_steps, net_path=a.net, mach=tuple(a.mach))
     print(json.dumps(rep))
 
 
