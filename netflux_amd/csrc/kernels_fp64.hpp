@@ -915,7 +915,10 @@ __global__ __launch_bounds__(B) NF_SWEEPB_ATTR void k_sweepB(DevMesh m, SweepBAr
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) R[k] = R[k] + flip(F[k], mneg ^ 0x80000000u);  // R_i -= F, R_j += F
-        __builtin_amdgcn_sched_barrier(0);
+        // the face's flux is complete here: without this the compiler sinks
+        // every face's arithmetic below the last face's loads, and all four
+        // faces' values are live at once
+        asm volatile("" : "+v"(R[0]), "+v"(R[1]), "+v"(R[2]), "+v"(R[3]), "+v"(R[4]) : : "memory");
     }
     if (live && a.check) {
         bool ok = true;

@@ -15,6 +15,16 @@
 namespace nf {
 namespace mx {
 
+// waves per SIMD the mixed sweep B is compiled for (0: the compiler's choice)
+#ifndef NF_SWEEPB_MX_WAVES
+#define NF_SWEEPB_MX_WAVES 0
+#endif
+#if NF_SWEEPB_MX_WAVES
+#define NF_SWEEPB_MX_ATTR __attribute__((amdgpu_waves_per_eu(NF_SWEEPB_MX_WAVES, NF_SWEEPB_MX_WAVES)))
+#else
+#define NF_SWEEPB_MX_ATTR
+#endif
+
 // kernel arguments
 struct GhostArgsMx {
     const uint32_t *bface;
@@ -274,7 +284,7 @@ __global__ __launch_bounds__(B) void k_sweepA_mx(DevMesh m, SweepAArgsMx a) {
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
+__global__ __launch_bounds__(B) NF_SWEEPB_MX_ATTR void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
     __shared__ double sh[B / 64];
     __shared__ double lds[12 * B];
     const LaneCell L = lane_cell<B>(m, a.xcd_remap);
@@ -300,9 +310,10 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
             Uop[k] = uo[tix(c, k)];
         }
     }
+    // face values of both sides, one face at a time, the next face's loads
+    // issued before the current face is computed (as the FP64 sweep)
     float Vo4[4][5], Vn4[4][5];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    auto load_face = [&](int s) {
         const uint32_t idx = w[s] & 0x1FFFFFFFu, sn = (w[s] >> 29) & 3u;
         const bool bnd = w[s] & 0x80000000u;
         const float *src = bnd ? a.Gf + tix(idx, 0) : a.FV + fix(idx, sn, 0);
@@ -311,13 +322,16 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
             Vo4[s][k] = a.FV[fix(c, s, k)];
             Vn4[s][k] = src[k * TILE];
         }
-    }
+    };
+    load_face(0);
     CellGeo g;
     const NodeView nv = node_table_finish<B>(m, L.lb * B + threadIdx.x, ntl, lds, g);
     cell_geo(m, g);
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
+        if (s + 1 < 4) load_face(s + 1);
+        __builtin_amdgcn_sched_barrier(0);
         const uint32_t byte = (code >> (8 * s)) & 0xFFu;
         V3f S;
         face_geo_mx(nv, g, byte, S, nullptr);
@@ -328,6 +342,10 @@ __global__ __launch_bounds__(B) void k_sweepB_mx(DevMesh m, SweepBArgsMx a) {
         rusanov_side_f(UO, UN, S, mneg, F);
 #pragma unroll
         for (int k = 0; k < 5; ++k) R[k] += (double)fflip(F[k], mneg ^ 0x80000000u);
+        // the face's flux is complete here: without this the compiler sinks
+        // every face's arithmetic below the last face's loads, and all four
+        // faces' values are live at once
+        asm volatile("" : "+v"(R[0]), "+v"(R[1]), "+v"(R[2]), "+v"(R[3]), "+v"(R[4]) : : "memory");
     }
     if (live && a.check) {
         bool ok = true;
