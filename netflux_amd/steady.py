@@ -437,6 +437,35 @@ def time_to_drop(net, n_cases=8, seed=7, cells_lo=2e4, cells_hi=1e5, drop=1e-8, 
     }
 
 
+def c5_scaled(net_path, n_cases=2, cells=1.5e6, seed=11, drop=1e-6, max_steps=400000, check_every=500,
+              precision="fp64"):
+    """The learned warm start at the inference resolution of the learned
+    workload: `n_cases` unseen bump channels of about `cells` cells (Mach
+    U[0.3, 0.7], h U[0.02, 0.1]), batched, local time steps, solved to `drop`
+    from the freestream and from the network's prediction in the same mode;
+    the warm start's own cost (features, inference, state upload) is timed
+    separately as well as inside the solve."""
+    net, meta = load_network(net_path)
+    cases = sample_cases(n_cases, seed, cells, cells, mach=(0.3, 0.7))
+    run = BatchRun(cases, precision=precision)
+    t0 = time.perf_counter()
+    for i, c in enumerate(cases):
+        run.batch.warm_start(i, net, (c.mach, c.h))
+    run.batch.get_solution(0)  # synchronises
+    t_warm = time.perf_counter() - t0
+    st_fs, wall_fs = run.solve(drop, max_steps, check_every)
+    st_w, wall_w = run.solve(drop, max_steps, check_every, net=net, stall_window=20000, stall_ratio=0.5)
+    return {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "precision": precision,
+            "network": {"file": net_path, "trained_on_cells": meta.get("cells"), "samples": meta.get("samples")},
+            "warm_start_s": t_warm,
+            "freestream": {"wall_s": wall_fs, "steps": [int(s["steps_done"]) for s in st_fs],
+                           "converged": [int(s["converged"]) for s in st_fs]},
+            "learned": {"wall_s": wall_w, "steps": [int(s["steps_done"]) for s in st_w],
+                        "converged": [int(s["converged"]) for s in st_w],
+                        "fell_back": [int(s["fell_back"]) for s in st_w]},
+            "gain": wall_fs / wall_w}
+
+
 def c3(drop=1e-8, max_steps=500000, check_every=200):
     """The 2M-tet channel (160 x 64 x 32, Mach 0.5, h = 0.05): FP64 with the
     reference's global time step against mixed precision with local steps --
@@ -492,9 +521,15 @@ def main(argv=None):
     ap.add_argument("--mach", type=float, nargs=2, default=(0.3, 0.8))
     ap.add_argument("--partial", default=None, help="write the report after each phase to this file")
     ap.add_argument("--c3", action="store_true", help="the 2M-tet mixed-precision channel case only")
+    ap.add_argument("--c5-scaled", action="store_true",
+                    help="unseen cases at the learned workload's inference size, freestream vs learned start (--net)")
     ap.add_argument("--train-save", default=None, metavar="PATH",
                     help="only train the warm-start network (train-cases at train-cells) and save it")
     a = ap.parse_args(argv)
+    if a.c5_scaled:
+        print(json.dumps(c5_scaled(a.net, n_cases=a.cases, cells=a.cells_hi, drop=a.drop, max_steps=a.max_steps,
+                                   check_every=a.check_every)))
+        return
     if a.train_save:
         net, info = train_from_solves(a.train_cases, a.train_cells, a.seed + 1000, a.per_case,
                                       max_steps=a.max_steps, check_every=a.check_every, epochs=a.epochs)
