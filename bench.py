@@ -342,6 +342,13 @@ def main():
     if args.steady_cases and args.precision == "fp64":
         del solver  # frees the 128^3 context first
         steady = steady_block(args)
+        if cpu and "plain_fp64" in steady:
+            # the reference's serial method on one host core, extrapolated: the
+            # plain GPU solve's step count of each case x 3 stages x its cells /
+            # the oracle's measured cell-stage-updates/s (same scheme except the
+            # limiter, so an estimate)
+            est = [n * 3 * c / cpu["value"] for n, c in zip(steady["plain_fp64"]["steps"], steady["cells"])]
+            steady["cpu_extrapolated_per_instance_s"] = sum(est) / len(est)
 
     out = {
         "metric": "cell-stage-updates/s",
