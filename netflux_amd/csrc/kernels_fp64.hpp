@@ -268,6 +268,11 @@ __device__ __forceinline__ void ghost_d(int bc, const double *Vd, V3d S, const F
 // of x + eps is monotone in x, so the largest positive Delta_2 is
 // max(d) + eps and the most negative is min(d) - eps (= -(|min d| + eps));
 // maxNum/minNum against 0 drop NaN and the values of the other sign.
+// A division whose quotient is at least 1 is skipped: for b > 0, a >= b
+// gives a / b >= 1 and, 1 being representable, round(a / b) >= 1, so
+// min(1, .) is 1 either way (for b < 0 the same holds with a <= b).  NaN
+// operands fail both comparisons and give 1, as minNum(1, NaN) does.  The
+// division then runs only in waves with a lane that limits.
 __device__ __forceinline__ double barth_psi_d(const double *delta, double vmax, double vmin, double v) {
     double hi = 0.0, lo = 0.0;
 #pragma unroll
@@ -275,8 +280,10 @@ __device__ __forceinline__ double barth_psi_d(const double *delta, double vmax, 
         hi = dmax(hi, delta[s]);
         lo = dmin(lo, delta[s]);
     }
-    const double pp = (hi > 0.0) ? dmin(1.0, (vmax - v) / (hi + DBL_EPSILON)) : 1.0;
-    const double pm = (lo < 0.0) ? dmin(1.0, (vmin - v) / (lo - DBL_EPSILON)) : 1.0;
+    const double ap = vmax - v, bp = hi + DBL_EPSILON, am = vmin - v, bm = lo - DBL_EPSILON;
+    double pp = 1.0, pm = 1.0;
+    if (hi > 0.0 && ap < bp) pp = dmin(1.0, ap / bp);
+    if (lo < 0.0 && am > bm) pm = dmin(1.0, am / bm);
     return dmin(pp, pm);
 }
 

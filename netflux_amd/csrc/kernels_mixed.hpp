@@ -129,7 +129,8 @@ __device__ __forceinline__ void rusanov_side_f(const float *UO, const float *UN,
 // Barth limiter of one variable over the four faces with one division per
 // sign group of Delta_2 (the FP64 kernels' barth_psi_d in FP32: the face
 // with the largest |Delta_2| of a group gives the group's smallest quotient,
-// and correctly rounded division is monotone in the divisor)
+// and correctly rounded division is monotone in the divisor); a division
+// whose quotient is at least 1 is skipped, as in barth_psi_d
 __device__ __forceinline__ float barth_psi_f(const float *delta, float vmax, float vmin, float v) {
     float hi = 0.0f, lo = 0.0f;
 #pragma unroll
@@ -137,8 +138,10 @@ __device__ __forceinline__ float barth_psi_f(const float *delta, float vmax, flo
         hi = fmaxf(hi, delta[s]);
         lo = fminf(lo, delta[s]);
     }
-    const float pp = (hi > 0.0f) ? fminf(1.0f, (vmax - v) / (hi + FLT_EPSILON)) : 1.0f;
-    const float pm = (lo < 0.0f) ? fminf(1.0f, (vmin - v) / (lo - FLT_EPSILON)) : 1.0f;
+    const float ap = vmax - v, bp = hi + FLT_EPSILON, am = vmin - v, bm = lo - FLT_EPSILON;
+    float pp = 1.0f, pm = 1.0f;
+    if (hi > 0.0f && ap < bp) pp = fminf(1.0f, ap / bp);
+    if (lo < 0.0f && am > bm) pm = fminf(1.0f, am / bm);
     return fminf(pp, pm);
 }
 
