@@ -26,6 +26,7 @@
 #include <new>
 #include <vector>
 
+#include "fp64_math.hpp"
 #include "physics.hpp"
 #include "poly.hpp"
 
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(PB) void k_poly_prim(const double *U, double *V, ui
     if (c >= N || stopped(stop)) return;
     double u[5], v[5];
     for (int k = 0; k < 5; ++k) u[k] = U[k * (size_t)N + c];
-    cons_to_prim(u, v);
+    cons_to_prim_d(u, v);
     for (int k = 0; k < 5; ++k) V[k * (size_t)NT + c] = v[k];
 }
 
@@ -78,69 +79,84 @@ __global__ __launch_bounds__(PB) void k_poly_ghost(PolyDev m, double *V, FarFiel
 }
 
 // Green-Gauss (Numerics.hpp:86-128), max/min (:253-292), Barth (:148-250)
-// and the reconstructed face values (Solver.hpp:81-112) of one cell
+// and the reconstructed face values (Solver.hpp:81-112) of one cell.  The
+// gradient terms divide by the cell volume through one refined reciprocal
+// (rdiv: the quotient of a plain division), the Barth limiter takes two
+// divisions per variable over the largest positive and most negative face
+// Delta_2 (barth_psi_hilo), and the face-centroid vectors are read once.
 __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, int limiter, double *FV,
                                                     double *grad, double *psi) {
     const uint32_t c = blockIdx.x * PB + threadIdx.x;
     if (c >= m.N || stopped(m.stop)) return;
     const size_t NT = m.NT, N = m.N;
     double vc[5];
+#pragma unroll
     for (int k = 0; k < 5; ++k) vc[k] = V[k * NT + c];
-    const double vol = m.vol[c];
+    const Rcp ivol = rcp_of(m.vol[c]);
     uint32_t w[6];
     int nfc = 0;
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
         w[s] = m.w[s * N + c];
         if (w[s] != NONE) nfc = s + 1;
     }
     double g[15];
+#pragma unroll
     for (int q = 0; q < 15; ++q) g[q] = 0.0;
     double vmax[5], vmin[5];
+#pragma unroll
     for (int k = 0; k < 5; ++k) vmax[k] = -DBL_MAX, vmin[k] = DBL_MAX;
-    for (int s = 0; s < nfc; ++s) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (s >= nfc) break;
         const uint32_t n = w[s] & IDX;
         const bool sj = (w[s] & SIDE_J) != 0;
         const double Sx = geo(m, s, 0, c), Sy = geo(m, s, 1, c), Sz = geo(m, s, 2, c);
+#pragma unroll
         for (int k = 0; k < 5; ++k) {
             const double vn = V[k * NT + n];
             const double vi = sj ? vn : vc[k], vj = sj ? vc[k] : vn;  // the face's i and j sides
             const double uf = 0.5 * (vi + vj);
-            const double t0 = uf * Sx, t1 = uf * Sy, t2 = uf * Sz;
+            const double t0 = rdiv(uf * Sx, ivol), t1 = rdiv(uf * Sy, ivol), t2 = rdiv(uf * Sz, ivol);
             if (!sj) {
-                g[k] += t0 / vol;
-                g[k + 5] += t1 / vol;
-                g[k + 10] += t2 / vol;
+                g[k] += t0;
+                g[k + 5] += t1;
+                g[k + 10] += t2;
             } else {
-                g[k] -= t0 / vol;
-                g[k + 5] -= t1 / vol;
-                g[k + 10] -= t2 / vol;
+                g[k] -= t0;
+                g[k + 5] -= t1;
+                g[k + 10] -= t2;
             }
             vmax[k] = smax(vmax[k], smax(vi, vj));
             vmin[k] = smin(vmin[k], smin(vi, vj));
         }
     }
+    V3d r[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        r[s] = s < nfc ? V3d{geo(m, s, 3, c), geo(m, s, 4, c), geo(m, s, 5, c)} : V3d{0.0, 0.0, 0.0};
     double ps[5];
-    if (limiter) {
-        for (int k = 0; k < 5; ++k) ps[k] = DBL_MAX;
-        for (int s = 0; s < nfc; ++s) {
-            const double rx = geo(m, s, 3, c), ry = geo(m, s, 4, c), rz = geo(m, s, 5, c);
-            for (int k = 0; k < 5; ++k) {
-                double D = (g[k] * rx + g[k + 5] * ry) + g[k + 10] * rz;
-                D = (double)isign(D) * (std::fabs(D) + DBL_EPSILON);
-                double fcon;
-                if (D > 0.0) fcon = smin(1.0, (vmax[k] - vc[k]) / D);
-                else if (D < 0.0) fcon = smin(1.0, (vmin[k] - vc[k]) / D);
-                else fcon = 1.0;
-                ps[k] = smin(ps[k], fcon);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        ps[k] = 1.0;  // SolverData.cpp:17 (no limiter)
+        if (limiter) {
+            double hi = 0.0, lo = 0.0;
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                if (s >= nfc) break;
+                const double D = (g[k] * r[s].x + g[k + 5] * r[s].y) + g[k + 10] * r[s].z;
+                hi = dmax(hi, D);
+                lo = dmin(lo, D);
             }
+            ps[k] = barth_psi_hilo(hi, lo, vmax[k], vmin[k], vc[k]);
         }
-    } else {
-        for (int k = 0; k < 5; ++k) ps[k] = 1.0;  // SolverData.cpp:17
     }
-    for (int s = 0; s < nfc; ++s) {
-        const double rx = geo(m, s, 3, c), ry = geo(m, s, 4, c), rz = geo(m, s, 5, c);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (s >= nfc) break;
+#pragma unroll
         for (int k = 0; k < 5; ++k) {
-            const double D = (g[k] * rx + g[k + 5] * ry) + g[k + 10] * rz;
+            const double D = (g[k] * r[s].x + g[k + 5] * r[s].y) + g[k + 10] * r[s].z;
             FV[((size_t)s * 5 + k) * N + c] = vc[k] + ps[k] * D;
         }
     }
@@ -178,18 +194,24 @@ __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
         const uint32_t n = w & IDX;
         const uint32_t q = m.ns[s * N + c];
         const V3 S = {geo(m, s, 0, c), geo(m, s, 1, c), geo(m, s, 2, c)};
-        double own[5], oth[5], ul[5], ur[5], F[5];
+        double own[5], oth[5], UO[5], UN[5], F[5];
         for (int k = 0; k < 5; ++k) own[k] = a.FV ? a.FV[((size_t)s * 5 + k) * N + c] : a.V[k * NT + c];
         if (w & BND) {  // the cell is the face's i side; ghost of its face value
-            ghost_value<double>((int)q, own, S, a.fs, oth);
+            ghost_d((int)q, own, S, a.fs, oth);
         } else {
             for (int k = 0; k < 5; ++k) oth[k] = a.FV ? a.FV[((size_t)q * 5 + k) * N + n] : a.V[k * NT + n];
         }
         const bool sj = (w & SIDE_J) != 0;
-        prim_to_cons(sj ? oth : own, ul);
-        prim_to_cons(sj ? own : oth, ur);
-        if (a.flux == 1) hllc_d(ul, ur, S, F);
-        else rusanov<double>(ul, ur, S, F);
+        prim_to_cons(own, UO);
+        prim_to_cons(oth, UN);
+        if (a.flux == 1) {
+            double ul[5], ur[5];  // the face's i and j sides
+#pragma unroll
+            for (int k = 0; k < 5; ++k) ul[k] = sj ? UN[k] : UO[k], ur[k] = sj ? UO[k] : UN[k];
+            hllc_d(ul, ur, S, F);
+        } else {
+            rusanov_side(UO, UN, S, sj ? 0x80000000u : 0u, F);  // = rusanov(ul, ur, S)
+        }
         if (sj) {
             for (int k = 0; k < 5; ++k) R[k] += F[k];
         } else {
