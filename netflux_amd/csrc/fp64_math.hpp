@@ -1,7 +1,8 @@
-// FP64 device arithmetic shared by the tetrahedral and the mixed-element
-// kernels: vector helpers, maxNum/minNum, exact sign flips, division by a
-// reused divisor, cons->prim, the Rusanov flux, boundary ghosts and the Barth
-// limiter, each giving the bits of the reference's plain FP64 formulas.
+// Device helpers shared by the tetrahedral and the mixed-element FP64
+// kernels: the XCD-aware block order, vector helpers, maxNum/minNum, exact
+// sign flips, division by a reused divisor, cons->prim, the Rusanov flux,
+// boundary ghosts and the Barth limiter, each giving the bits of the
+// reference's plain FP64 formulas.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,6 +12,15 @@
 #include "physics.hpp"
 
 namespace nf {
+
+// Logical block index: blocks are dispatched round robin over the 8 XCDs;
+// give each XCD a contiguous range of cells so that neighbour gathers hit the
+// L2 of the XCD that touched the same region.
+__device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t nb, int xcd_remap) {
+    if (!xcd_remap) return b;
+    const uint32_t x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
 
 using V3d = Vec3T<double>;
 

@@ -114,15 +114,6 @@ struct DevMesh {
     InstState *st;             // [instances]
 };
 
-// Logical block index: blocks are dispatched round robin over the 8 XCDs;
-// give each XCD a contiguous range of cells so that neighbour gathers hit the
-// L2 of the XCD that touched the same region.
-__device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t nb, int xcd_remap) {
-    if (!xcd_remap) return b;
-    const uint32_t x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
-    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 // The lane's cell, its instance, and whether it is a real cell.  Dead lanes
 // of the last block of an instance mirror the instance's last cell and store
 // nothing.

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(PB) void k_poly_ghost(PolyDev m, double *V, FarFiel
 // Delta_2 (barth_psi_hilo), and the face-centroid vectors are read once.
 __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, int limiter, double *FV,
                                                     double *grad, double *psi) {
-    const uint32_t c = blockIdx.x * PB + threadIdx.x;
+    const uint32_t c = logical_block(blockIdx.x, gridDim.x, 1) * PB + threadIdx.x;  // XCD-contiguous
     if (c >= m.N || stopped(m.stop)) return;
     const size_t NT = m.NT, N = m.N;
     double vc[5];
@@ -108,13 +108,18 @@ __global__ __launch_bounds__(PB) void k_poly_sweepA(PolyDev m, const double *V, 
     for (int k = 0; k < 5; ++k) vmax[k] = -DBL_MAX, vmin[k] = DBL_MAX;
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        if (s >= nfc) break;
-        const uint32_t n = w[s] & IDX;
-        const bool sj = (w[s] & SIDE_J) != 0;
+        // loads before the slot test (absent slots read the cell's own
+        // entries, in bounds), so that later faces' loads need not wait
+        const uint32_t n = s < nfc ? (w[s] & IDX) : c;
         const double Sx = geo(m, s, 0, c), Sy = geo(m, s, 1, c), Sz = geo(m, s, 2, c);
+        double vnb[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vnb[k] = V[k * NT + n];
+        if (s >= nfc) break;
+        const bool sj = (w[s] & SIDE_J) != 0;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            const double vn = V[k * NT + n];
+            const double vn = vnb[k];
             const double vi = sj ? vn : vc[k], vj = sj ? vc[k] : vn;  // the face's i and j sides
             const double uf = 0.5 * (vi + vj);
             const double t0 = rdiv(uf * Sx, ivol), t1 = rdiv(uf * Sy, ivol), t2 = rdiv(uf * Sz, ivol);
@@ -184,24 +189,35 @@ struct StageArgs {
 // inviscid_fluxes (src/Solver.cpp:144-230) gathered per cell, the validity
 // check (src/SolverData.cpp:66-79) and the stage update (src/Solver.cpp:83-133)
 __global__ __launch_bounds__(PB) void k_poly_sweepB(PolyDev m, StageArgs a) {
-    const uint32_t c = blockIdx.x * PB + threadIdx.x;
+    const uint32_t c = logical_block(blockIdx.x, gridDim.x, 1) * PB + threadIdx.x;  // XCD-contiguous
     if (c >= m.N || stopped(m.stop)) return;
     const size_t NT = m.NT, N = m.N;
     double R[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // slot words first, then every face's loads without waiting on them;
+    // absent slots and boundary faces read the cell's own entries (in
+    // bounds) and are masked afterwards
+    uint32_t w[6], qs[6];
+    int nfc = 0;
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const uint32_t w = m.w[s * N + c];
-        if (w == NONE) break;
-        const uint32_t n = w & IDX;
-        const uint32_t q = m.ns[s * N + c];
+        w[s] = m.w[s * N + c];
+        qs[s] = m.ns[s * N + c];
+    }
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (w[s] != NONE) nfc = s + 1;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const bool inner = s < nfc && !(w[s] & BND);
+        const uint32_t n = inner ? (w[s] & IDX) : c, q = inner ? qs[s] : (uint32_t)s;
         const V3 S = {geo(m, s, 0, c), geo(m, s, 1, c), geo(m, s, 2, c)};
         double own[5], oth[5], UO[5], UN[5], F[5];
         for (int k = 0; k < 5; ++k) own[k] = a.FV ? a.FV[((size_t)s * 5 + k) * N + c] : a.V[k * NT + c];
-        if (w & BND) {  // the cell is the face's i side; ghost of its face value
-            ghost_d((int)q, own, S, a.fs, oth);
-        } else {
-            for (int k = 0; k < 5; ++k) oth[k] = a.FV ? a.FV[((size_t)q * 5 + k) * N + n] : a.V[k * NT + n];
-        }
-        const bool sj = (w & SIDE_J) != 0;
+        for (int k = 0; k < 5; ++k) oth[k] = a.FV ? a.FV[((size_t)q * 5 + k) * N + n] : a.V[k * NT + n];
+        if (s >= nfc) break;
+        if (w[s] & BND)  // the cell is the face's i side; ghost of its face value
+            ghost_d((int)qs[s], own, S, a.fs, oth);
+        const bool sj = (w[s] & SIDE_J) != 0;
         prim_to_cons(own, UO);
         prim_to_cons(oth, UN);
         if (a.flux == 1) {
