@@ -541,12 +541,13 @@ def main(argv=None):
     ap.add_argument("--c3", action="store_true", help="the 2M-tet mixed-precision channel case only")
     ap.add_argument("--c5-scaled", action="store_true",
                     help="unseen cases at the learned workload's inference size, freestream vs learned start (--net)")
+    ap.add_argument("--c5-seed", type=int, default=11, help="case seed of --c5-scaled")
     ap.add_argument("--train-save", default=None, metavar="PATH",
                     help="only train the warm-start network (train-cases at train-cells) and save it")
     a = ap.parse_args(argv)
     if a.c5_scaled:
-        print(json.dumps(c5_scaled(a.net, n_cases=a.cases, cells=a.cells_hi, drop=a.drop, max_steps=a.max_steps,
-                                   check_every=a.check_every)))
+        print(json.dumps(c5_scaled(a.net, n_cases=a.cases, cells=a.cells_hi, seed=a.c5_seed, drop=a.drop,
+                                   max_steps=a.max_steps, check_every=a.check_every)))
         return
     if a.train_save:
         net, info = train_from_solves(a.train_cases, a.train_cells, a.seed + 1000, a.per_case,
