@@ -9,7 +9,9 @@ Checks: two TVD-RK3 steps against the oracle (the reference's first-step
 zero-U_old quirk included), held to 1e-12 relative L2 (farfield faces use
 the device pow, 1e-13 level); a uniform freestream stays uniform to
 round-off; with slip walls the total mass and energy stay constant to
-round-off (each interior face flux enters its two cells exactly negated)."""
+round-off (each interior face flux enters its two cells exactly negated).
+The mixed-element path is checked against the oracle at the size its own
+benchmark times (96^3 hexahedra)."""
 import math
 
 import numpy as np
@@ -93,3 +95,36 @@ def test_benchmark_mesh_slip_walls_conserve():
         m0, m1 = math.fsum(vol * U0[:, k]), math.fsum(vol * U[:, k])  # exact sums of the products
         print(f"variable {k}: total {m0:.17g} -> {m1:.17g}, relative drift {abs(m1 - m0) / m0:.2e}", flush=True)
         assert abs(m1 - m0) / m0 < 1e-12
+
+
+def test_hex_benchmark_mesh_two_steps_vs_oracle():
+    """The mixed-element path at the size `bench.py --mesh hex` times: 96^3
+    jittered hexahedra (884,736 cells, six face slots each), Mach 0.5 +
+    pulse, farfield; two TVD-RK3 + Barth steps against the oracle's
+    mixed-element grid, held to 1e-12 relative L2 (farfield faces use the
+    device pow), with the same time steps."""
+    mesh = nf.Mesh.hex_box(96, jitter=0.15, seed=SEED, patches=("FarField",) * 6)
+    print(f"[case] hex 96^3 = {mesh.n_cells} cells, pulse, 2 RK3 steps vs oracle", flush=True)
+    assert mesh.n_cells == 884_736
+    U0 = pulse_state(mesh)
+    cfg = pulse_config(mach=0.5)
+    g = nf.EulerSolver(cfg, mesh)
+    g.set_state(U0)
+    dts = []
+    for _ in range(2):
+        g.calc_timestep(cfg)
+        dts.append(cfg.delta_time)
+        g.step(cfg)
+    Ug = g.get_solver_data().get_solution()
+    del g
+    print("[case] GPU steps done; oracle", flush=True)
+    o = Oracle.from_mesh(mesh, pulse_config(mach=0.5))
+    o.set_state(U0)
+    dto = []
+    for _ in range(2):
+        dto.append(o.calc_timestep())
+        o.step(dto[-1])
+    err = rel_l2(Ug, o.get("U"))
+    print(f"dt gpu {dts} oracle {dto}; rel L2 {err:.3e}", flush=True)
+    assert dts == dto
+    assert err < 1e-12
