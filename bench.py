@@ -54,11 +54,12 @@ def bytes_model_mixed(N, F):
     return sweep1, sweep2_s1, sweep2_s23, step
 
 
-def profiled_traffic(kernel, n, fname="sweep_traffic_128cubed.md"):
+def profiled_traffic(kernel, n, fname="sweep_traffic_128cubed.md", profiled_n=128):
     """HBM bytes per launch of `kernel` from the committed PMC summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the 128^3 workload."""
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the workload of size
+    profiled_n^3 (None at other sizes)."""
     path = os.path.join(ROOT, "profiles", fname)
-    if n != 128 or not os.path.exists(path):
+    if n != profiled_n or not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
@@ -202,8 +203,10 @@ def bench_hex(args, nf, pulse_config, pulse_state):
     if per_launch:
         achieved = per_launch / (kern[dom]["avg_ms"] / 1e3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "traffic_note": "PMC upper bounds in profiles/poly_kernels_hex96.md",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": profiled_traffic(dom, n, "poly_kernels_hex96.md", profiled_n=96),
+                    "traffic_source": "profiles/poly_kernels_hex96.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1; "
+                                      "an upper bound for scattered reads)",
                     "kernel": dom, "kernel_avg_ms": round(kern[dom]["avg_ms"], 4),
                     "algorithmic_bytes_per_launch": int(per_launch)}
     out = {
