@@ -252,6 +252,49 @@ def load_network(path):
     return net, meta
 
 
+def solve_training_chunk(train_cases, train_cells, seed, per_case=4000, train_drop=1e-6, max_steps=60000,
+                         check_every=200):
+    """One chunk of training data: steady FP64 solutions of `train_cases`
+    cases sampled with `seed` (batched, to `train_drop`) and `per_case`
+    samples of each converged one; returns (x, y, description)."""
+    tcases = sample_cases(train_cases, seed, train_cells, train_cells)
+    trun = BatchRun(tcases, precision="fp64")
+    tst, twall = trun.solve(train_drop, max_steps, check_every)
+    x, y = training_set(trun, per_case, tst, seed)
+    del trun
+    info = {"cases": train_cases, "cells": int(tcases[0].n_cells), "case_seed": seed, "drop": train_drop,
+            "solves": _summary(tst), "solve_wall_s": twall, "samples": int(len(x))}
+    _log(f"[train data] {info}")
+    return x.astype(np.float32), y.astype(np.float32), info
+
+
+def save_training_chunk(path, x, y, info):
+    import json
+    np.savez(path, x=x, y=y, meta=np.array(json.dumps(info)))
+
+
+def train_from_chunks(paths, epochs=200):
+    """A network trained by the library's kernels on the samples of saved
+    training chunks (plain arrays, nothing unpickled)."""
+    import json
+    xs, ys, infos = [], [], []
+    for p in paths:
+        with np.load(p, allow_pickle=False) as z:
+            xs.append(z["x"])
+            ys.append(z["y"])
+            infos.append(json.loads(str(z["meta"])))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    t0 = time.perf_counter()
+    net, tstats = train_network(x, y, epochs=epochs)
+    _log(f"[train] {tstats}")
+    info = {"cases": int(sum(i["cases"] for i in infos)),
+            "converged": int(sum(i["solves"]["converged"] for i in infos)),
+            "cells": infos[0]["cells"], "case_seeds": [i["case_seed"] for i in infos], "drop": infos[0]["drop"],
+            "solve_wall_s": float(sum(i["solve_wall_s"] for i in infos)), "samples": int(len(x)),
+            "epochs": epochs, "train": tstats, "train_s": time.perf_counter() - t0, "chunks": infos}
+    return net, info
+
+
 def train_from_solves(train_cases, train_cells, seed, per_case=4000, train_drop=1e-6, max_steps=60000,
                       check_every=200, epochs=200):
     """Steady FP64 solutions of `train_cases` sampled cases of about
@@ -542,12 +585,28 @@ def main(argv=None):
     ap.add_argument("--c5-scaled", action="store_true",
                     help="unseen cases at the learned workload's inference size, freestream vs learned start (--net)")
     ap.add_argument("--c5-seed", type=int, default=11, help="case seed of --c5-scaled")
+    ap.add_argument("--train-chunk", default=None, metavar="PATH",
+                    help="only solve one chunk of training cases (train-cases at train-cells, case seed "
+                         "--seed) and save its samples")
+    ap.add_argument("--train-from", nargs="+", default=None, metavar="PATH",
+                    help="train on saved chunks (with --train-save PATH for the network)")
     ap.add_argument("--train-save", default=None, metavar="PATH",
                     help="only train the warm-start network (train-cases at train-cells) and save it")
     a = ap.parse_args(argv)
     if a.c5_scaled:
         print(json.dumps(c5_scaled(a.net, n_cases=a.cases, cells=a.cells_hi, seed=a.c5_seed, drop=a.drop,
                                    max_steps=a.max_steps, check_every=a.check_every)))
+        return
+    if a.train_chunk:
+        x, y, info = solve_training_chunk(a.train_cases, a.train_cells, a.seed, a.per_case,
+                                          max_steps=a.max_steps, check_every=a.check_every)
+        save_training_chunk(a.train_chunk, x, y, info)
+        print(json.dumps(info))
+        return
+    if a.train_from:
+        net, info = train_from_chunks(a.train_from, epochs=a.epochs)
+        save_network(net, a.train_save, info)
+        print(json.dumps(info))
         return
     if a.train_save:
         net, info = train_from_solves(a.train_cases, a.train_cells, a.seed + 1000, a.per_case,
