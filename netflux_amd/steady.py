@@ -499,13 +499,14 @@ def time_to_drop(net, n_cases=8, seed=7, cells_lo=2e4, cells_hi=1e5, drop=1e-8, 
 
 
 def c5_scaled(net_path, n_cases=2, cells=1.5e6, seed=11, drop=1e-6, max_steps=400000, check_every=500,
-              precision="fp64"):
+              precision="fp64", freestream=True):
     """The learned warm start at the inference resolution of the learned
     workload: `n_cases` unseen bump channels of about `cells` cells (Mach
     U[0.3, 0.7], h U[0.02, 0.1]), batched, local time steps, solved to `drop`
     from the freestream and from the network's prediction in the same mode;
     the warm start's own cost (features, inference, state upload) is timed
-    separately as well as inside the solve."""
+    separately as well as inside the solve.  freestream=False skips the
+    freestream solve (it does not depend on the network)."""
     net, meta = load_network(net_path)
     cases = sample_cases(n_cases, seed, cells, cells, mach=(0.3, 0.7))
     run = BatchRun(cases, precision=precision)
@@ -514,17 +515,20 @@ def c5_scaled(net_path, n_cases=2, cells=1.5e6, seed=11, drop=1e-6, max_steps=40
         run.batch.warm_start(i, net, (c.mach, c.h))
     run.batch.get_solution(0)  # synchronises
     t_warm = time.perf_counter() - t0
-    st_fs, wall_fs = run.solve(drop, max_steps, check_every)
+    rep = {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "precision": precision,
+           "network": {"file": net_path, "trained_on_cells": meta.get("cells"), "samples": meta.get("samples")},
+           "warm_start_s": t_warm}
+    if freestream:
+        st_fs, wall_fs = run.solve(drop, max_steps, check_every)
+        rep["freestream"] = {"wall_s": wall_fs, "steps": [int(s["steps_done"]) for s in st_fs],
+                             "converged": [int(s["converged"]) for s in st_fs]}
     st_w, wall_w = run.solve(drop, max_steps, check_every, net=net, stall_window=20000, stall_ratio=0.5)
-    return {"cases": [c.__dict__ | {"cells": c.n_cells} for c in cases], "drop": drop, "precision": precision,
-            "network": {"file": net_path, "trained_on_cells": meta.get("cells"), "samples": meta.get("samples")},
-            "warm_start_s": t_warm,
-            "freestream": {"wall_s": wall_fs, "steps": [int(s["steps_done"]) for s in st_fs],
-                           "converged": [int(s["converged"]) for s in st_fs]},
-            "learned": {"wall_s": wall_w, "steps": [int(s["steps_done"]) for s in st_w],
-                        "converged": [int(s["converged"]) for s in st_w],
-                        "fell_back": [int(s["fell_back"]) for s in st_w]},
-            "gain": wall_fs / wall_w}
+    rep["learned"] = {"wall_s": wall_w, "steps": [int(s["steps_done"]) for s in st_w],
+                      "converged": [int(s["converged"]) for s in st_w],
+                      "fell_back": [int(s["fell_back"]) for s in st_w]}
+    if freestream:
+        rep["gain"] = wall_fs / wall_w
+    return rep
 
 
 def c3(drop=1e-8, max_steps=500000, check_every=200):
@@ -585,6 +589,7 @@ def main(argv=None):
     ap.add_argument("--c5-scaled", action="store_true",
                     help="unseen cases at the learned workload's inference size, freestream vs learned start (--net)")
     ap.add_argument("--c5-seed", type=int, default=11, help="case seed of --c5-scaled")
+    ap.add_argument("--c5-no-freestream", action="store_true", help="--c5-scaled: the warm-started solve only")
     ap.add_argument("--train-chunk", default=None, metavar="PATH",
                     help="only solve one chunk of training cases (train-cases at train-cells, case seed "
                          "--seed) and save its samples")
@@ -595,7 +600,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.c5_scaled:
         print(json.dumps(c5_scaled(a.net, n_cases=a.cases, cells=a.cells_hi, seed=a.c5_seed, drop=a.drop,
-                                   max_steps=a.max_steps, check_every=a.check_every)))
+                                   max_steps=a.max_steps, check_every=a.check_every,
+                                   freestream=not a.c5_no_freestream)))
         return
     if a.train_chunk:
         x, y, info = solve_training_chunk(a.train_cases, a.train_cells, a.seed, a.per_case,
