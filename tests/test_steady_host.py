@@ -47,3 +47,19 @@ def test_floor_cells_touch_the_bump():
     yb = 0.08 * np.exp(-((g["centroid"][owner, 0] - 1.5) ** 2) / (2 * 0.2 ** 2))
     assert np.all(g["centroid"][owner, 1] - yb < 1.0 / case.ny)
     np.testing.assert_allclose(p, 1 / 1.4, rtol=1e-14)
+
+
+def test_training_chunk_round_trip(tmp_path):
+    """A saved chunk of training samples loads back as plain arrays (no
+    pickles) with its description."""
+    import json
+    rng = np.random.default_rng(0)
+    x = rng.random((10, 6)).astype(np.float32)
+    y = rng.random((10, 5)).astype(np.float32)
+    info = {"cases": 2, "case_seed": 5, "samples": 10, "solves": {"converged": 2}}
+    p = tmp_path / "chunk.npz"
+    S.save_training_chunk(p, x, y, info)
+    with np.load(p, allow_pickle=False) as z:
+        np.testing.assert_array_equal(z["x"], x)
+        np.testing.assert_array_equal(z["y"], y)
+        assert json.loads(str(z["meta"])) == info
