@@ -610,6 +610,8 @@ def main(argv=None):
         print(json.dumps(info))
         return
     if a.train_from:
+        if not a.train_save:
+            ap.error("--train-from needs --train-save PATH for the network")
         net, info = train_from_chunks(a.train_from, epochs=a.epochs)
         save_network(net, a.train_save, info)
         print(json.dumps(info))
