@@ -323,3 +323,34 @@ def test_oblique_supersonic_farfield_matches_oracle():
     print("rel L2 after 10, 40, 80 steps:", errs)
     assert max(errs) <= TOL_REL_L2
     assert np.abs(ora.get("U") - U0).max() > 1e-3  # the flow has moved
+
+
+def test_single_tetrahedron_and_ragged_box():
+    """Ragged extremes of the tetrahedral sweeps: one cell in a 256-lane block
+    (255 padding lanes, every face a boundary face, no interior face so the
+    reference's time step is DBL_MAX), and a 3 x 2 x 5 box (180 cells: the
+    block's third wave is partly padding) stepped 20 times against the
+    oracle, bit for bit."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.1, 0.2, 0.9]], float)
+    cells = np.array([[0, 1, 2, 3]], np.uint32)
+    faces = [[[0, 2, 1]], [[0, 1, 3]], [[0, 3, 2]], [[1, 2, 3]]]
+    one = nf.Mesh.from_arrays(nodes, cells, faces, ["SlipWall", "FarField", "SlipWall", "FarField"])
+    cfg = pulse_config(mach=0.4)
+    gpu, ora = _pair(one, cfg, pulse_state(one))
+    gpu.calc_timestep(cfg)
+    assert cfg.delta_time == ora.calc_timestep() == np.finfo(float).max
+    gpu.eval_flux_balance()
+    ora.eval_flux_balance()
+    R_g, R_o = gpu.get_solver_data().get_flux_balance(), ora.get("R")
+    assert np.all(np.isfinite(R_g)) and np.max(np.abs(R_g - R_o)) <= 1e-13 * np.max(np.abs(R_o))
+    box = nf.Mesh.box((3, 2, 5), lengths=(0.6, 0.4, 1.0), jitter=0.15, seed=5)
+    assert box.n_cells == 180
+    cfg = sod_config()
+    gpu, ora = _pair(box, cfg, sod_state(box))
+    for _ in range(20):
+        gpu.calc_timestep(cfg)
+        dt = ora.calc_timestep()
+        assert cfg.delta_time == dt
+        gpu.step(cfg)
+        ora.step(dt)
+    np.testing.assert_array_equal(gpu.get_solver_data().get_solution(), ora.get("U"))
