@@ -51,3 +51,27 @@ def test_committed_network_loads_and_predicts_physical_states():
     assert run.batch.warm_start(0, net, (case.mach, case.h)) is False
     V = SD.cons_to_prim(run.batch.get_solution(0))
     assert (V[:, 0] > 0).all() and (V[:, 4] > 0).all()
+
+
+def test_training_in_chunks(tmp_path):
+    """Training data solved in chunks (one GPU job each for large sets) and
+    a network trained on the saved chunks: every converged case contributes
+    its samples, the loss falls, and predictions are physical."""
+    print("[case] two chunks of 4 channels at 6000 cells, trained from the saved chunks", flush=True)
+    paths = []
+    for seed in (301, 302):
+        x, y, info = SD.solve_training_chunk(4, 6000, seed, per_case=2000, max_steps=40000, check_every=200)
+        assert x.shape == (2000 * info["solves"]["converged"], 6) and y.shape == (len(x), 5)
+        assert info["solves"]["converged"] >= 3
+        p = tmp_path / f"chunk_{seed}.npz"
+        SD.save_training_chunk(p, x, y, info)
+        paths.append(p)
+    net, info = SD.train_from_chunks(paths, epochs=40)
+    print("training:", info["samples"], info["train"], flush=True)
+    assert info["cases"] == 8 and info["samples"] == 2000 * info["converged"]
+    assert info["train"]["loss_last"] < 0.1 * info["train"]["loss_first"]
+    case = SD.sample_cases(1, seed=303, cells_lo=6000, cells_hi=6000)[0]
+    run = SD.BatchRun([case], precision="fp64")
+    assert not run.batch.warm_start(0, net, (case.mach, case.h))  # no fallback: a physical prediction
+    V = SD.cons_to_prim(run.batch.get_solution(0))
+    assert np.all(V[:, 0] > 0) and np.all(V[:, 4] > 0)
