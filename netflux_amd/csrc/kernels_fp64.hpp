@@ -801,6 +801,27 @@ __global__ __launch_bounds__(BLOCK) void k_prim(const double *U, double *V, uint
 #pragma unroll
     for (int k = 0; k < 5; ++k) V[tix(c, k)] = v[k];
 }
+
+// State across the boundary: reference-order AoS [N][5] (copied to the
+// device as is) to the instance's device tiles, padding cells zero; perm is
+// the instance's device cell -> reference cell map.
+__global__ __launch_bounds__(BLOCK) void k_state_in(const double *aos, const uint32_t *perm, double *dev,
+                                                    uint32_t N, uint32_t Npad) {
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= Npad) return;
+    const size_t r = p < N ? 5 * (size_t)perm[p] : 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dev[tix(p, k)] = p < N ? aos[r + k] : 0.0;
+}
+// and back: device tiles to reference-order AoS
+__global__ __launch_bounds__(BLOCK) void k_state_out(const double *dev, const uint32_t *perm, double *aos,
+                                                     uint32_t N) {
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= N) return;
+    const size_t r = 5 * (size_t)perm[p];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) aos[r + k] = dev[tix(p, k)];
+}
 #endif  // NF_DEVICE_HELPERS_ONLY
 
 }  // namespace nf

@@ -247,6 +247,8 @@ struct nf_ctx {
     float *feat = nullptr, *vpred = nullptr;        // network features / prediction scratch
     uint32_t feat_cap = 0;
     double *r_eval = nullptr;                       // [instances]
+    double *stage = nullptr;                        // [5 x largest instance] AoS state staging
+    uint32_t *perm_dev = nullptr;                   // [Ncell] device cell -> reference cell of its instance
     bool aux_valid = false;
     std::vector<void *> owned;
     hipGraphExec_t graph = nullptr;
@@ -560,16 +562,47 @@ int ensure_hist(nf_ctx *ctx, int need) {
 size_t tile_index(uint32_t c, int k) { return ((size_t)(c >> 6) * 5 + k) * TILE + (c & 63u); }
 
 // reference-order AoS [N][5] of instance i -> device tiles
+// the staging buffer and the device copy of the cell maps, made on first use
+int ensure_stage(nf_ctx *ctx) {
+    if (ctx->stage) return NF_OK;
+    uint32_t nmax = 1;
+    std::vector<uint32_t> perm(ctx->Ncell, 0);
+    for (const Inst &I : ctx->inst) {
+        nmax = std::max(nmax, I.N);
+        std::copy(I.perm.begin(), I.perm.end(), perm.begin() + I.cell0);
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->perm_dev, ctx->Ncell))) return rc;
+    HIPC(hipMemcpy(ctx->perm_dev, perm.data(), perm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return dalloc(ctx, &ctx->stage, (size_t)5 * nmax);
+}
+
+// reference-order AoS [N][5] of instance i -> its device tiles: the array
+// crosses as is and the device reorders it (the host never touches it)
 int upload_state(nf_ctx *ctx, int i, const double *aos, double *dev) {
     const Inst &I = ctx->inst[i];
-    std::vector<double> d((size_t)5 * I.Npad, 0.0);
-    for (uint32_t c = 0; c < I.N; ++c)
-        for (int k = 0; k < 5; ++k) d[tile_index(c, k)] = aos[5 * (size_t)I.perm[c] + k];
-    HIPC(hipMemcpy(dev + (size_t)5 * I.cell0, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = ensure_stage(ctx))) return rc;
+    HIPC(hipMemcpyAsync(ctx->stage, aos, (size_t)5 * I.N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_state_in, dim3(I.Npad / BLOCK), dim3(BLOCK), 0, ctx->stream, (const double *)ctx->stage,
+                       (const uint32_t *)(ctx->perm_dev + I.cell0), dev + (size_t)5 * I.cell0, I.N, I.Npad);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(ctx->stream));
     return NF_OK;
 }
 
 // device tiles of instance i -> reference-order AoS [N][5]
+int download_state(nf_ctx *ctx, int i, const double *dev, double *aos) {
+    const Inst &I = ctx->inst[i];
+    int rc;
+    if ((rc = ensure_stage(ctx))) return rc;
+    hipLaunchKernelGGL(k_state_out, dim3(I.Npad / BLOCK), dim3(BLOCK), 0, ctx->stream,
+                       dev + (size_t)5 * I.cell0, (const uint32_t *)(ctx->perm_dev + I.cell0), ctx->stage, I.N);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(aos, ctx->stage, (size_t)5 * I.N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return NF_OK;
+}
 template <typename T>
 int download_state(nf_ctx *ctx, int i, const T *dev, double *aos) {
     const Inst &I = ctx->inst[i];
