@@ -374,6 +374,21 @@ __global__ __launch_bounds__(PB) void k_poly_copy(double *dst, const double *src
     if (q < n && !stopped(stop)) dst[q] = src[q];
 }
 
+// reference-order AoS [N][cols] <-> [cols][N] in device cell order
+__global__ __launch_bounds__(PB) void k_poly_aos_in(const double *aos, const uint32_t *perm, double *soa, uint32_t N,
+                                                    int cols) {
+    const uint32_t d = blockIdx.x * PB + threadIdx.x;
+    if (d >= N) return;
+    const size_t r = (size_t)cols * perm[d];
+    for (int k = 0; k < cols; ++k) soa[k * (size_t)N + d] = aos[r + k];
+}
+__global__ __launch_bounds__(PB) void k_poly_aos_out(const double *soa, const uint32_t *perm, double *aos, uint32_t N,
+                                                     int cols) {
+    const uint32_t d = blockIdx.x * PB + threadIdx.x;
+    if (d >= N) return;
+    const size_t r = (size_t)cols * perm[d];
+    for (int k = 0; k < cols; ++k) aos[r + k] = soa[k * (size_t)N + d];
+}
 __global__ __launch_bounds__(PB) void k_poly_fill(double *p, double v, size_t n) {
     const size_t q = (size_t)blockIdx.x * PB + threadIdx.x;
     if (q < n) p[q] = v;
@@ -404,6 +419,8 @@ struct PolyPath {
     std::vector<std::pair<const char *, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::map<std::string, std::pair<double, int>> ktime;
     std::vector<uint32_t> perm;  // device cell -> reference cell
+    uint32_t *perm_dev = nullptr;  // the same on the device (made on first state transfer)
+    double *stage = nullptr;       // [15][N] AoS staging of state and field transfers
     std::vector<double> hist;    // residual norm of each step (flux balance of U^n)
     double time = 0.0;
     bool converged = false;
@@ -437,22 +454,35 @@ template <typename T> int upload(T *dst, const std::vector<T> &src, std::string 
 dim3 grid(size_t n) { return dim3((unsigned)((n + PB - 1) / PB)); }
 
 // AoS [N][cols] in reference cell order <-> SoA [cols][N] in device order
+// Reference-order AoS [N][cols] crosses as is; the device reorders it into
+// [cols][N] in device cell order (and back), the host never touches it.
+int ensure_stage(PolyPath *P, std::string &err) {
+    if (P->stage) return NF_OK;
+    int rc;
+    if ((rc = alloc(P, &P->perm_dev, P->m.N, err)) || (rc = alloc(P, &P->stage, 15 * (size_t)P->m.N, err))) return rc;
+    PHIP(hipMemcpy(P->perm_dev, P->perm.data(), (size_t)P->m.N * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return NF_OK;
+}
 int put_soa(PolyPath *P, double *dst, const double *aos, int cols, std::string &err) {
     const uint32_t N = P->m.N;
-    std::vector<double> soa((size_t)cols * N);
-    for (uint32_t d = 0; d < N; ++d)
-        for (int k = 0; k < cols; ++k) soa[k * (size_t)N + d] = aos[(size_t)cols * P->perm[d] + k];
-    PHIP(hipMemcpyAsync(dst, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, P->st));
+    int rc;
+    if ((rc = ensure_stage(P, err))) return rc;
+    PHIP(hipMemcpyAsync(P->stage, aos, (size_t)cols * N * 8, hipMemcpyHostToDevice, P->st));
+    hipLaunchKernelGGL(k_poly_aos_in, grid(N), dim3(PB), 0, P->st, (const double *)P->stage,
+                       (const uint32_t *)P->perm_dev, dst, N, cols);
+    PHIP(hipGetLastError());
     PHIP(hipStreamSynchronize(P->st));
     return NF_OK;
 }
 int get_soa(PolyPath *P, const double *src, int cols, double *aos, std::string &err) {
     const uint32_t N = P->m.N;
-    std::vector<double> soa((size_t)cols * N);
+    int rc;
+    if ((rc = ensure_stage(P, err))) return rc;
+    hipLaunchKernelGGL(k_poly_aos_out, grid(N), dim3(PB), 0, P->st, src, (const uint32_t *)P->perm_dev, P->stage,
+                       N, cols);
+    PHIP(hipGetLastError());
+    PHIP(hipMemcpyAsync(aos, P->stage, (size_t)cols * N * 8, hipMemcpyDeviceToHost, P->st));
     PHIP(hipStreamSynchronize(P->st));
-    PHIP(hipMemcpy(soa.data(), src, soa.size() * 8, hipMemcpyDeviceToHost));
-    for (uint32_t d = 0; d < N; ++d)
-        for (int k = 0; k < cols; ++k) aos[(size_t)cols * P->perm[d] + k] = soa[k * (size_t)N + d];
     return NF_OK;
 }
 
